@@ -1,0 +1,270 @@
+"""fug30v0kxc_amd -- MI355X-native per-timestep core of the p3d compressible
+magnetoconvection solver (reference "VarElecCond", Fortran 90).
+
+The package is the host-side mirror of the reference's time-step interface.
+The reference exposes argument-less module subroutines acting on module
+state (src/main.F90:340-747): hydro_deriv, mag_deriv, bandj, magdealias,
+boundary, mag_boundary, GlobalAverages, nusselt, checkdivb, sminmax, calcdt,
+plus the operators d1bydx, d1bydy, d2bydx, d2bydy, dealias, forwardfft,
+inversefft (src/fft.F90) and d1bydz, d1upbydz, d2bydz (src/dbydz.F90).
+`Solver` keeps those names and argument meanings; every call goes through
+the C ABI (include/mhd.h) to hand-written gfx950 kernels.  Errors carry the
+reference's abort / iret codes (MhdError.code).
+
+Arrays are numpy float64 of shape (nz, ny, nx) in C order, which is the
+reference's Fortran (nx, ny, nz) column-major layout.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FIELD_IDS, HISTORY_FIELDS, KERNELS, MHD_PREC_FP64, MHD_PREC_MIXED, OPS,
+                   VECTOR_FIELDS, MhdDiag, MhdParams, MhdScalars)
+
+__all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "MHD_PREC_FP64",
+           "MHD_PREC_MIXED"]
+
+
+class MhdError(RuntimeError):
+    """A non-zero return code of the C ABI (the reference's abort codes)."""
+
+    def __init__(self, code, what):
+        msg = _abi.load().mhd_strerror(code).decode()
+        super().__init__(f"{what}: {msg} (code {code})")
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def ra_q_to_ck_f(gamma, sigma, theta, cm, zeta, ra, q):
+    """INPUT flag 2: Rayleigh and Chandrasekhar numbers -> CK, F
+    (src/initial.F90:183-195)."""
+    ck = ctypes.c_double()
+    f = ctypes.c_double()
+    _abi.load().mhd_ra_q_to_ck_f(gamma, sigma, theta, cm, zeta, ra, q, ctypes.byref(ck),
+                                 ctypes.byref(f))
+    return ck.value, f.value
+
+
+def reference_params(ra=1e5, q=100.0, t_amp=1e-3, b_amp=0.0, gamma=5.0 / 3.0, sigma=1.0,
+                     theta=10.0, cm=1.0, zeta=0.2, xmax=4.0, ymax=4.0, sf=0.2, tayl=0.0,
+                     psi=1.570796327):
+    """Parameter dict of the reference's INPUT (flag 2) with Ra, Q converted."""
+    ck, f = ra_q_to_ck_f(gamma, sigma, theta, cm, zeta, ra, q)
+    return dict(gamma=gamma, sigma=sigma, theta=theta, ck=ck, cm=cm, f=f, tayl=tayl, psi=psi,
+                zeta=zeta, xmax=xmax, ymax=ymax, sf=sf, t_amp=t_amp, b_amp=b_amp)
+
+
+class Solver:
+    """One box of the solver on one GPU (one C-ABI context)."""
+
+    def __init__(self, nx, ny, nz, params, precision=MHD_PREC_FP64):
+        self._lib = _abi.load()
+        self._ctx = None
+        p = MhdParams(**params)
+        ctx = ctypes.c_void_p()
+        self._chk(self._lib.mhd_create(ctypes.byref(p), nx, ny, nz, precision,
+                                       ctypes.byref(ctx)), "mhd_create")
+        self._ctx = ctx
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (nz, ny, nx)
+        self.params = dict(params)
+        self.precision = precision
+
+    def close(self):
+        if self._ctx is not None:
+            self._lib.mhd_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def _chk(code, what):
+        if code != 0:
+            raise MhdError(code, what)
+
+    # -- state ------------------------------------------------------------
+    def static(self):
+        """static() + ini_trans_params() + mag_init() (src/main.F90:275-333)."""
+        self._chk(self._lib.mhd_static(self._ctx), "static")
+
+    def set_state(self, r, t, u, v, w, bx, by, bz):
+        arrs = [_f64(a, self.shape) for a in (r, t, u, v, w, bx, by, bz)]
+        self._chk(self._lib.mhd_set_state(self._ctx, *[_dp(a) for a in arrs]), "set_state")
+
+    def get_state(self):
+        """dict of the 8 dumped fields (R, T, U, V, W, Bx, By, Bz)."""
+        names = ("r", "t", "u", "v", "w", "bx", "by", "bz")
+        out = {n: np.empty(self.shape) for n in names}
+        self._chk(self._lib.mhd_get_state(self._ctx, *[_dp(out[n]) for n in names]),
+                  "get_state")
+        return out
+
+    def field(self, name, level=3):
+        n = self.nz if name in VECTOR_FIELDS else None
+        out = np.empty(n) if n else np.empty(self.shape)
+        self._chk(self._lib.mhd_get_field(self._ctx, FIELD_IDS[name], level, _dp(out)),
+                  f"get_field({name})")
+        return out
+
+    def set_field(self, name, value, level=3):
+        shape = (self.nz,) if name in VECTOR_FIELDS else self.shape
+        a = _f64(value, shape)
+        self._chk(self._lib.mhd_set_field(self._ctx, FIELD_IDS[name], level, _dp(a)),
+                  f"set_field({name})")
+
+    def scalars(self):
+        s = MhdScalars()
+        self._chk(self._lib.mhd_get_scalars(self._ctx, ctypes.byref(s)), "get_scalars")
+        return {n: getattr(s, n) for n, _ in MhdScalars._fields_}
+
+    def set_scalars(self, **kw):
+        s = MhdScalars()
+        self._chk(self._lib.mhd_get_scalars(self._ctx, ctypes.byref(s)), "get_scalars")
+        for k, v in kw.items():
+            setattr(s, k, v)
+        self._chk(self._lib.mhd_set_scalars(self._ctx, ctypes.byref(s)), "set_scalars")
+
+    # -- the reference's per-step routines ----------------------------------
+    def hydro_deriv(self):
+        self._chk(self._lib.mhd_hydro_deriv(self._ctx), "hydro_deriv")
+
+    def mag_deriv(self):
+        self._chk(self._lib.mhd_mag_deriv(self._ctx), "mag_deriv")
+
+    def bandj(self):
+        self._chk(self._lib.mhd_bandj(self._ctx), "bandj")
+
+    def magdealias(self):
+        self._chk(self._lib.mhd_magdealias(self._ctx), "magdealias")
+
+    def boundary(self):
+        self._chk(self._lib.mhd_boundary(self._ctx), "boundary")
+
+    def mag_boundary(self):
+        self._chk(self._lib.mhd_mag_boundary(self._ctx), "mag_boundary")
+
+    def mag_init(self):
+        self._chk(self._lib.mhd_mag_init(self._ctx), "mag_init")
+
+    def step(self, nsteps, dt_fixed=0.0, ndu06=0):
+        """nsteps passes of the time-loop body (src/main.F90:340-683).
+        Returns the last diagnostics as a dict (NaN-free only if computed)."""
+        d = MhdDiag()
+        d.nit = -1
+        self._chk(self._lib.mhd_step(self._ctx, nsteps, dt_fixed, ndu06, ctypes.byref(d)),
+                  "step")
+        return {n: getattr(d, n) for n, _ in MhdDiag._fields_}
+
+    # -- diagnostics (src/utils.F90) ----------------------------------------
+    def GlobalAverages(self):
+        """(ubar, vbar, kenergy, menergy); subtracts ubar, vbar from u, v."""
+        out = np.zeros(4)
+        self._chk(self._lib.mhd_global_averages(self._ctx, _dp(out)), "GlobalAverages")
+        return tuple(out)
+
+    def nusselt(self):
+        v = ctypes.c_double()
+        self._chk(self._lib.mhd_nusselt(self._ctx, ctypes.byref(v)), "nusselt")
+        return v.value
+
+    def checkdivb(self):
+        out = np.zeros(2)
+        self._chk(self._lib.mhd_checkdivb(self._ctx, _dp(out)), "checkdivb")
+        return tuple(out)
+
+    def sminmax(self, name):
+        out = np.zeros(2)
+        self._chk(self._lib.mhd_sminmax(self._ctx, FIELD_IDS[name], _dp(out)), "sminmax")
+        return tuple(out)
+
+    def calcdt(self):
+        v = ctypes.c_double()
+        self._chk(self._lib.mhd_calcdt(self._ctx, ctypes.byref(v)), "calcdt")
+        return v.value
+
+    # -- operators (src/fft.F90, src/dbydz.F90) -----------------------------
+    def _op(self, name, fin, flag=0, two=False, vec=False):
+        shape = (self.nz,) if vec else self.shape
+        a = _f64(fin, shape)
+        o1 = np.empty(shape)
+        o2 = np.empty(shape) if two else None
+        self._chk(self._lib.mhd_op(self._ctx, OPS[name], _dp(a), _dp(o1),
+                                   _dp(o2) if two else None, flag), name)
+        return (o1, o2) if two else o1
+
+    def d1bydx(self, fin):
+        return self._op("d1bydx", fin)
+
+    def d1bydy(self, fin):
+        return self._op("d1bydy", fin)
+
+    def d2bydx(self, fin):
+        """(first, second) x-derivative, like d2bydx(fin, f1out, f2out)."""
+        return self._op("d2bydx", fin, two=True)
+
+    def d2bydy(self, fin):
+        return self._op("d2bydy", fin, two=True)
+
+    def dealias(self, z):
+        return self._op("dealias", z)
+
+    def forwardfft(self, fin):
+        return self._op("forwardfft", fin)
+
+    def inversefft(self, fin):
+        return self._op("inversefft", fin)
+
+    def d1bydz(self, fin, dflag):
+        return self._op("d1bydz", fin, dflag)
+
+    def d1upbydz(self, fin, dflag):
+        """Upwind d/dz selected by the solver's own w (as the reference)."""
+        return self._op("d1upbydz", fin, dflag)
+
+    def d2bydz(self, fin, dflag):
+        return self._op("d2bydz", fin, dflag)
+
+    def d1bydz1D(self, fin, dflag):
+        return self._op("d1bydz1D", fin, dflag, vec=True)
+
+    def d2bydz1D(self, fin, dflag):
+        return self._op("d2bydz1D", fin, dflag, vec=True)
+
+    # -- measurement ------------------------------------------------------
+    def kernel_timing(self, kernel, enable=True):
+        self._chk(self._lib.mhd_kernel_timing(self._ctx, KERNELS[kernel], int(enable)),
+                  "kernel_timing")
+
+    def kernel_time(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        self._chk(self._lib.mhd_kernel_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)),
+                  "kernel_time")
+        return ms.value, n.value
+
+    def sync(self):
+        self._chk(self._lib.mhd_sync(self._ctx), "sync")
+
+    def device_bytes(self):
+        return self._lib.mhd_device_bytes(self._ctx)

@@ -1,0 +1,1274 @@
+// mhd_api.cpp -- host side of the C ABI (include/mhd.h): context, device
+// memory, and the time-loop driver that sequences the gfx950 kernels in the
+// order of the reference's src/main.F90:340-683.
+//
+// Device layout (one hipMalloc pool per context, every array 256-B aligned):
+//   state (FP64):   u v w t r pol tor, bxbar bybar (nz)
+//   B and J^2:      bx by bz jsq (working precision)
+//   RHS history:    7 fields x 3 levels (FP64), ring-indexed so the
+//                   reference's level shift (2->1, 3->2) is a relabelling
+//   work:           derivative planes of the RHS assembly (working precision)
+//   tables:         akx1 akx2 aky1 aky2, kappa dkappa, FFT twiddles
+#include "../../include/mhd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "physics.hpp"
+#include "spectral.hpp"
+#include "zstencil.hpp"
+
+using namespace mhd;
+
+int mhd_hip_error(hipError_t e, const char* what, const char* file, int line)
+{
+    fprintf(stderr, "mhd: HIP error %d (%s) at %s:%d: %s\n", (int)e, hipGetErrorString(e), file, line, what);
+    return MHD_E_DEVICE;
+}
+
+namespace {
+
+// work buffers of the RHS assembly
+enum Wk {
+    W_RX, W_TX, W_TXX, W_UX, W_UXX, W_VX, W_VXX, W_WX, W_WXX, W_XBB, W_XZX,
+    W_RY, W_TY, W_TYY, W_UY, W_UYY, W_VY, W_VYY, W_WY, W_WYY, W_YBB, W_YZY,
+    W_TZU, W_TZC, W_TZZ, W_WZU, W_WZC, W_UZU, W_UZZ, W_VZU, W_VZZ, W_ZZX, W_ZZY, W_RZU, W_RZC,
+    W_C1, W_C2, W_C3, W_C4, W_DXC1, W_DYC2, W_DZC3, W_DZC4,
+    W_COUNT
+};
+
+constexpr int MAXEV = 8192;
+
+}  // namespace
+
+struct mhd_ctx {
+    int nx, ny, nz, nxy;
+    long n3;
+    int prec, wf32;
+    size_t wsz;  // bytes per working element
+    mhd_params_t p;
+    Phys ph;
+    Geom g;
+    int icutx, icuty;
+    double dx, dy, dz, dd;
+    hipStream_t s;
+
+    char* pool;
+    size_t pool_bytes;
+    double *akx1, *akx2, *aky1, *aky2, *kappa, *dkappa;
+    void *twx, *twy;
+    double *u, *v, *w, *t, *r, *pol, *tor, *bxbar, *bybar;
+    void *bx, *by, *bz, *jsq;
+    double* hist[7][3];
+    double* mhist[2][3];
+    int ring[3];
+    void* wk[W_COUNT];
+    double *red_partial, *red_result, *d_bzbar;
+    double* stage;     // device FP64 staging (one field)
+    double* hbuf;      // pinned host staging (one field)
+
+    mhd_scalars_t sc;
+    int have_state;
+
+    int timed_kernel;
+    int nev;
+    hipEvent_t ev[2 * MAXEV];
+    int ev_created;
+};
+
+namespace {
+
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// history field index (u v w r t pol tor) for MHD_F_* ids
+int hist_index(int id)
+{
+    switch (id) {
+    case MHD_F_UDOT: return 0;
+    case MHD_F_VDOT: return 1;
+    case MHD_F_WDOT: return 2;
+    case MHD_F_RDOT: return 3;
+    case MHD_F_TDOT: return 4;
+    case MHD_F_POLDOT: return 5;
+    case MHD_F_TORDOT: return 6;
+    }
+    return -1;
+}
+
+double* level_ptr(mhd_ctx* c, int f, int level) { return c->hist[f][c->ring[level - 1]]; }
+double* mlevel_ptr(mhd_ctx* c, int f, int level) { return c->mhist[f][c->ring[level - 1]]; }
+
+void set_constants(mhd_ctx* c)
+{
+    // src/constants.f90:56-109, src/dbydz.F90:355-362
+    const mhd_params_t& p = c->p;
+    Phys& h = c->ph;
+    c->dx = p.xmax / (double)c->nx;
+    c->dy = p.ymax / (double)c->ny;
+    c->dz = 1.0 / (double)(c->nz - 1);
+    c->dd = std::fmin(c->dx, c->dy);
+    const double taylt = p.tayl / std::pow(1.0 + p.theta / 2.0, 2.0 * p.cm);
+    h.cpsi = std::cos(p.psi);
+    h.spsi = std::sin(p.psi);
+    h.cor = p.ck * p.sigma * std::sqrt(taylt);
+    h.gck = p.gamma * p.ck;
+    h.sck = p.sigma * p.ck;
+    h.zck = p.zeta * p.ck;
+    h.scko3 = h.sck / 3.0;
+    h.sckg = p.sigma * p.ck * (p.gamma - 1.0);
+    h.fzckg = p.f * p.zeta * p.ck * (p.gamma - 1.0);
+    h.thmplus1 = p.theta * (p.cm + 1.0);
+    h.fo2 = p.f / 2.0;
+    h.g1 = std::sqrt(p.gamma);
+    h.gm1 = p.gamma - 1.0;
+    h.nusstmp = p.theta * (p.cm + 1.0) * (1.0 - (1.0 / p.gamma));
+    h.f = p.f;
+    h.theta = p.theta;
+    h.xn = 1.0 / (double)(c->nx * c->ny);
+    h.xzn = c->dx * c->dy * c->dz;
+    h.dz = c->dz;
+    h.dd = c->dd;
+    h.sf = p.sf;
+    h.dtvisc0x = 0.1 * 0.75 * c->dd * c->dd / p.sigma / p.ck;
+    h.dtvisc0z = 0.2 * c->dz * c->dz / p.sigma / p.ck * (1.0 + p.theta / 7.0);
+    h.dtdiff0x = 0.1 * c->dd * c->dd / p.gamma / p.ck;
+    h.dtdiff0z = 0.2 * c->dz * c->dz / p.gamma / p.ck * (1.0 + p.theta / 7.0);
+    h.ho12 = (double)(c->nz - 1) / 12.0;
+    h.sc1 = (double)((c->nz - 1) * (c->nz - 1)) / 12.0;
+    h.mag = (p.f != 0.0);
+}
+
+// src/fft.F90:171-221
+void wavenumbers(int n, double len, std::vector<double>& a1, std::vector<double>& a2)
+{
+    a1.assign(n, 0.0);
+    a2.assign(n, 0.0);
+    const double cx = 8.0 * std::atan(1.0) / len;
+    for (int i = 1; i <= n / 2 - 1; ++i) {
+        a1[i] = (double)i * cx;
+        a1[n - i] = (double)i * cx;
+        a2[i] = -((double)i * (double)i * cx * cx);
+        a2[n - i] = -((double)i * (double)i * cx * cx);
+    }
+    a2[n / 2] = -((double)(n / 2) * (double)(n / 2) * cx * cx);
+}
+
+// W_N^t = exp(-2 pi i t / N), rounded once from long double
+void twiddles(int n, std::vector<double>& d, std::vector<float>& f)
+{
+    d.resize(2 * n);
+    f.resize(2 * n);
+    for (int t = 0; t < n; ++t) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+        d[2 * t] = (double)cosl(a);
+        d[2 * t + 1] = (double)sinl(a);
+        f[2 * t] = (float)cosl(a);
+        f[2 * t + 1] = (float)sinl(a);
+    }
+}
+
+// src/trans_params.f90:36-62
+void kappa_profile(int nz, double dz, std::vector<double>& ka, std::vector<double>& dka)
+{
+    ka.resize(nz);
+    dka.resize(nz);
+    const double constA = 10.0;
+    for (int k = 0; k < nz; ++k) {
+        const double z = (double)k * dz;
+        const double constB = constA * (z - 5.0e-1);
+        ka[k] = 5.0e-1 * (std::tanh(constB) + 1);
+        dka[k] = 5.0e-1 * constA * std::pow(std::cosh(constB), -2.0);
+    }
+}
+
+// Park-Miller minimal standard with Schrage's factorisation and the XOR mask,
+// src/ranset.f90:16-29
+double ran0(int* idum)
+{
+    const int IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836, MASK = 123459876;
+    const double AM = 1.0 / IM;
+    *idum ^= MASK;
+    const int k = *idum / IQ;
+    *idum = IA * (*idum - k * IQ) - IR * k;
+    if (*idum < 0) *idum += IM;
+    const double r = AM * (*idum);
+    *idum ^= MASK;
+    return r;
+}
+
+SpecLaunch spec_base(mhd_ctx* c, int dir, int mode)
+{
+    SpecLaunch L;
+    memset(&L, 0, sizeof(L));
+    L.g = c->g;
+    L.dir = dir;
+    L.mode = mode;
+    L.ak1 = dir == 0 ? c->akx1 : c->aky1;
+    L.ak2 = dir == 0 ? c->akx2 : c->aky2;
+    L.tw = dir == 0 ? c->twx : c->twy;
+    L.icut = dir == 0 ? c->icutx : c->icuty;
+    return L;
+}
+
+SpecJob sjob(const void* in, int in_f32, void* o0, void* o1, int m0, int m1, int out_f32)
+{
+    SpecJob j;
+    memset(&j, 0, sizeof(j));
+    j.in[0] = in;
+    j.in_f32 = in_f32;
+    j.load_op = LD_PLAIN;
+    j.scale = 1.0;
+    j.out[0] = o0;
+    j.out[1] = o1;
+    j.mul[0] = m0;
+    j.mul[1] = m1;
+    j.out_f32 = out_f32;
+    return j;
+}
+
+ZJob zjob(const void* in, int in_f32, void* out, int out_f32, int kind, int flag)
+{
+    ZJob j;
+    memset(&j, 0, sizeof(j));
+    j.in = in;
+    j.in_f32 = in_f32;
+    j.load_op = 0;
+    j.scale = 1.0;
+    j.out = out;
+    j.out_f32 = out_f32;
+    j.kind = kind;
+    j.flag = flag;
+    return j;
+}
+
+ZLaunch zbase(mhd_ctx* c)
+{
+    ZLaunch L;
+    memset(&L, 0, sizeof(L));
+    L.g = c->g;
+    L.ho12 = c->ph.ho12;
+    L.sc1 = c->ph.sc1;
+    L.w = c->w;
+    L.w_f32 = 0;
+    return L;
+}
+
+// timed launch wrapper
+template <typename F>
+int timed(mhd_ctx* c, int kernel, F&& launch)
+{
+    const bool on = (c->timed_kernel == kernel) && c->nev + 2 <= 2 * MAXEV;
+    if (on && hipEventRecord(c->ev[c->nev], c->s) != hipSuccess) return MHD_E_DEVICE;
+    int e = launch();
+    if (on) {
+        if (hipEventRecord(c->ev[c->nev + 1], c->s) != hipSuccess) return MHD_E_DEVICE;
+        c->nev += 2;
+    }
+    return e;
+}
+
+int spec(mhd_ctx* c, const SpecLaunch& L, int kernel = MHD_K_NONE)
+{
+    const int cf32 = c->wf32;
+    int e = timed(c, kernel, [&] { return launch_spectral(L, cf32, c->s); });
+    return e ? (e == -20 ? MHD_E_SIZE : MHD_E_DEVICE) : 0;
+}
+
+int zrun(mhd_ctx* c, const ZLaunch& L)
+{
+    int e = timed(c, MHD_K_ZDERIV, [&] { return launch_z(L, c->s); });
+    return e == 7 ? MHD_E_FLAG : (e ? MHD_E_DEVICE : 0);
+}
+
+HydroArgs hargs(mhd_ctx* c)
+{
+    HydroArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = c->g;
+    a.p = c->ph;
+    a.u = c->u; a.v = c->v; a.w = c->w; a.t = c->t; a.r = c->r;
+    a.bx = c->bx; a.by = c->by; a.bz = c->bz; a.jsq = c->jsq;
+    a.kappa = c->kappa; a.dkappa = c->dkappa;
+    void** W = c->wk;
+    a.rx = W[W_RX]; a.tx = W[W_TX]; a.txx = W[W_TXX]; a.ux = W[W_UX]; a.uxx = W[W_UXX];
+    a.vx = W[W_VX]; a.vxx = W[W_VXX]; a.wx = W[W_WX]; a.wxx = W[W_WXX]; a.xbb = W[W_XBB]; a.xzx = W[W_XZX];
+    a.ry = W[W_RY]; a.ty = W[W_TY]; a.tyy = W[W_TYY]; a.uy = W[W_UY]; a.uyy = W[W_UYY];
+    a.vy = W[W_VY]; a.vyy = W[W_VYY]; a.wy = W[W_WY]; a.wyy = W[W_WYY]; a.ybb = W[W_YBB]; a.yzy = W[W_YZY];
+    a.tzu = W[W_TZU]; a.tzc = W[W_TZC]; a.tzz = W[W_TZZ]; a.wzu = W[W_WZU]; a.wzc = W[W_WZC];
+    a.uzu = W[W_UZU]; a.uzz = W[W_UZZ]; a.vzu = W[W_VZU]; a.vzz = W[W_VZZ]; a.zzx = W[W_ZZX]; a.zzy = W[W_ZZY];
+    a.rzu = W[W_RZU]; a.rzc = W[W_RZC];
+    a.dxc1 = W[W_DXC1]; a.dyc2 = W[W_DYC2]; a.dzc3 = W[W_DZC3]; a.dzc4 = W[W_DZC4];
+    a.c1 = W[W_C1]; a.c2 = W[W_C2]; a.c3 = W[W_C3]; a.c4 = W[W_C4];
+    a.rdot = level_ptr(c, 3, 3);
+    a.tdot = level_ptr(c, 4, 3);
+    a.udot = level_ptr(c, 0, 3);
+    a.vdot = level_ptr(c, 1, 3);
+    a.wdot = level_ptr(c, 2, 3);
+    a.wf32 = c->wf32;
+    return a;
+}
+
+MagArgs margs(mhd_ctx* c)
+{
+    MagArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = c->g;
+    a.p = c->ph;
+    a.icutx = c->icutx;
+    a.icuty = c->icuty;
+    a.akx1 = c->akx1; a.akx2 = c->akx2; a.aky1 = c->aky1; a.aky2 = c->aky2;
+    a.pol = c->pol; a.tor = c->tor; a.bxbar = c->bxbar; a.bybar = c->bybar;
+    a.poldot = level_ptr(c, 5, 3);
+    a.tordot = level_ptr(c, 6, 3);
+    a.bxbardot = mlevel_ptr(c, 0, 3);
+    a.bybardot = mlevel_ptr(c, 1, 3);
+    void** W = c->wk;
+    // the RHS-assembly planes are free while the magnetic routines run
+    a.exh = W[W_RX]; a.eyh = W[W_TX]; a.ezh = W[W_TXX];
+    a.d = W[W_UX]; a.cc = W[W_UXX]; a.dz = W[W_VX];
+    a.polz = W[W_VXX];
+    a.bxh = W[W_WX]; a.byh = W[W_WXX]; a.bzh = W[W_XBB]; a.jzh = W[W_XZX]; a.dybzh = W[W_RY]; a.dxbzh = W[W_TY];
+    a.bx = c->bx; a.by = c->by; a.jsq = c->jsq;
+    a.wf32 = c->wf32;
+    return a;
+}
+
+int kerr(int e) { return e ? MHD_E_DEVICE : 0; }
+
+#define TRY(x)                    \
+    do {                          \
+        int _r = (x);             \
+        if (_r) return _r;        \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// hydro_deriv, src/hydro_deriv.f90:5-505
+// ---------------------------------------------------------------------------
+int run_hydro_deriv(mhd_ctx* c)
+{
+    const int wf = c->wf32;
+    void** W = c->wk;
+    const double f = c->p.f;
+    // x derivatives: first+second of t,u,v,w; first of r and of the two
+    // Maxwell-stress products F*by*bx, F*bz*bx
+    {
+        SpecLaunch L = spec_base(c, 0, SM_DERIV2);
+        L.job[0] = sjob(c->t, 0, W[W_TX], W[W_TXX], MUL_D1, MUL_D2, wf);
+        L.job[1] = sjob(c->u, 0, W[W_UX], W[W_UXX], MUL_D1, MUL_D2, wf);
+        L.job[2] = sjob(c->v, 0, W[W_VX], W[W_VXX], MUL_D1, MUL_D2, wf);
+        L.job[3] = sjob(c->w, 0, W[W_WX], W[W_WXX], MUL_D1, MUL_D2, wf);
+        L.njobs = 4;
+        TRY(spec(c, L, MHD_K_XDERIV2));
+        SpecLaunch M = spec_base(c, 0, SM_DERIV1);
+        M.job[0] = sjob(c->r, 0, W[W_RX], nullptr, MUL_D1, 0, wf);
+        M.job[1] = sjob(c->by, wf, W[W_XBB], nullptr, MUL_D1, 0, wf);
+        M.job[1].load_op = LD_PROD; M.job[1].in[1] = c->bx; M.job[1].scale = f;
+        M.job[2] = sjob(c->bz, wf, W[W_XZX], nullptr, MUL_D1, 0, wf);
+        M.job[2].load_op = LD_PROD; M.job[2].in[1] = c->bx; M.job[2].scale = f;
+        M.njobs = 3;
+        TRY(spec(c, M));
+    }
+    // y derivatives
+    {
+        SpecLaunch L = spec_base(c, 1, SM_DERIV2);
+        L.job[0] = sjob(c->t, 0, W[W_TY], W[W_TYY], MUL_D1, MUL_D2, wf);
+        L.job[1] = sjob(c->u, 0, W[W_UY], W[W_UYY], MUL_D1, MUL_D2, wf);
+        L.job[2] = sjob(c->v, 0, W[W_VY], W[W_VYY], MUL_D1, MUL_D2, wf);
+        L.job[3] = sjob(c->w, 0, W[W_WY], W[W_WYY], MUL_D1, MUL_D2, wf);
+        L.njobs = 4;
+        TRY(spec(c, L, MHD_K_YDERIV2));
+        SpecLaunch M = spec_base(c, 1, SM_DERIV1);
+        M.job[0] = sjob(c->r, 0, W[W_RY], nullptr, MUL_D1, 0, wf);
+        M.job[1] = sjob(c->by, wf, W[W_YBB], nullptr, MUL_D1, 0, wf);
+        M.job[1].load_op = LD_PROD; M.job[1].in[1] = c->bx; M.job[1].scale = f;
+        M.job[2] = sjob(c->bz, wf, W[W_YZY], nullptr, MUL_D1, 0, wf);
+        M.job[2].load_op = LD_PROD; M.job[2].in[1] = c->by; M.job[2].scale = f;
+        M.njobs = 3;
+        TRY(spec(c, M));
+    }
+    // z derivatives (flags as in the reference's calls)
+    {
+        ZLaunch L = zbase(c);
+        int n = 0;
+        L.job[n++] = zjob(c->t, 0, W[W_TZU], wf, ZK_D1UP, 5);  // 66
+        L.job[n++] = zjob(c->t, 0, W[W_TZC], wf, ZK_D1, 5);    // 67
+        L.job[n++] = zjob(c->t, 0, W[W_TZZ], wf, ZK_D2, 0);    // 68
+        L.job[n++] = zjob(c->w, 0, W[W_WZU], wf, ZK_D1UP, 1);  // 124
+        L.job[n++] = zjob(c->w, 0, W[W_WZC], wf, ZK_D1, 1);    // 125
+        L.job[n++] = zjob(c->u, 0, W[W_UZU], wf, ZK_D1UP, 2);  // 233
+        L.job[n++] = zjob(c->u, 0, W[W_UZZ], wf, ZK_D2, 3);    // 349
+        L.job[n++] = zjob(c->v, 0, W[W_VZU], wf, ZK_D1UP, 2);  // 275
+        L.job[n++] = zjob(c->v, 0, W[W_VZZ], wf, ZK_D2, 3);    // 350
+        L.job[n] = zjob(c->bz, wf, W[W_ZZX], wf, ZK_D1, 5);    // 272: F*bz*bx
+        L.job[n].load_op = LD_PROD; L.job[n].in1 = c->bx; L.job[n].scale = f; ++n;
+        L.job[n] = zjob(c->bz, wf, W[W_ZZY], wf, ZK_D1, 5);    // 321: F*bz*by
+        L.job[n].load_op = LD_PROD; L.job[n].in1 = c->by; L.job[n].scale = f; ++n;
+        L.job[n++] = zjob(c->r, 0, W[W_RZU], wf, ZK_D1UP, 0);  // 443
+        L.job[n++] = zjob(c->r, 0, W[W_RZC], wf, ZK_D1, 0);    // 444
+        L.njobs = n;
+        TRY(zrun(c, L));
+    }
+    HydroArgs a = hargs(c);
+    TRY(timed(c, MHD_K_HYDRO_A, [&] { return kerr(launch_hydro_a(a, c->s)); }));
+    // derivatives of the composite fluxes (181-184)
+    {
+        SpecLaunch L = spec_base(c, 0, SM_DERIV1);
+        L.job[0] = sjob(W[W_C1], wf, W[W_DXC1], nullptr, MUL_D1, 0, wf);
+        L.njobs = 1;
+        TRY(spec(c, L));
+        SpecLaunch M = spec_base(c, 1, SM_DERIV1);
+        M.job[0] = sjob(W[W_C2], wf, W[W_DYC2], nullptr, MUL_D1, 0, wf);
+        M.njobs = 1;
+        TRY(spec(c, M));
+        ZLaunch Z = zbase(c);
+        Z.job[0] = zjob(W[W_C3], wf, W[W_DZC3], wf, ZK_D1, 2);
+        Z.job[1] = zjob(W[W_C4], wf, W[W_DZC4], wf, ZK_D1, 5);
+        Z.njobs = 2;
+        TRY(zrun(c, Z));
+    }
+    return kerr(launch_hydro_b(a, c->s));
+}
+
+// ---------------------------------------------------------------------------
+// mag_deriv, src/magnetic.f90:285-487
+// ---------------------------------------------------------------------------
+int run_mag_deriv(mhd_ctx* c)
+{
+    const int wf = c->wf32;
+    MagArgs m = margs(c);
+    {
+        SpecLaunch L = spec_base(c, 0, SM_R2HC);
+        // E = u x B (308-316), fused into the x pass of forwardfft
+        L.job[0] = sjob(c->v, wf, m.exh, nullptr, 0, 0, wf);
+        L.job[0].load_op = LD_CROSS; L.job[0].in[1] = c->bz; L.job[0].in[2] = c->w; L.job[0].in[3] = c->by;
+        L.job[1] = sjob(c->w, wf, m.eyh, nullptr, 0, 0, wf);
+        L.job[1].load_op = LD_CROSS; L.job[1].in[1] = c->bx; L.job[1].in[2] = c->u; L.job[1].in[3] = c->bz;
+        L.job[2] = sjob(c->u, wf, m.ezh, nullptr, 0, 0, wf);
+        L.job[2].load_op = LD_CROSS; L.job[2].in[1] = c->by; L.job[2].in[2] = c->v; L.job[2].in[3] = c->bx;
+        L.njobs = 3;
+        if (wf) return MHD_E_STATE;  // mixed path wires FP32 velocity shadows here
+        TRY(spec(c, L));
+        SpecLaunch M = spec_base(c, 1, SM_R2HC);
+        M.job[0] = sjob(m.exh, wf, m.exh, nullptr, 0, 0, wf);
+        M.job[1] = sjob(m.eyh, wf, m.eyh, nullptr, 0, 0, wf);
+        M.job[2] = sjob(m.ezh, wf, m.ezh, nullptr, 0, 0, wf);
+        M.njobs = 3;
+        TRY(spec(c, M));
+    }
+    TRY(kerr(launch_mean_emf(m, c->s)));
+    TRY(kerr(launch_mag_spec1(m, c->s)));
+    {
+        ZLaunch Z = zbase(c);
+        Z.job[0] = zjob(m.d, wf, m.dz, wf, ZK_D1, 5);  // 419
+        Z.njobs = 1;
+        TRY(zrun(c, Z));
+    }
+    return kerr(launch_mag_rhs(m, c->s));
+}
+
+// ---------------------------------------------------------------------------
+// bandj, src/magnetic.f90:492-664 (jsq tail shared with mag_init)
+// ---------------------------------------------------------------------------
+int inverse_b_and_j(mhd_ctx* c, MagArgs& m, bool with_b)
+{
+    const int wf = c->wf32;
+    SpecLaunch L = spec_base(c, 0, SM_HC2R);
+    int n = 0;
+    if (with_b) {
+        L.job[n++] = sjob(m.bxh, wf, m.bxh, nullptr, 0, 0, wf);
+        L.job[n++] = sjob(m.byh, wf, m.byh, nullptr, 0, 0, wf);
+        L.job[n++] = sjob(m.bzh, wf, m.bzh, nullptr, 0, 0, wf);
+    }
+    L.job[n++] = sjob(m.dybzh, wf, m.dybzh, nullptr, 0, 0, wf);
+    L.job[n++] = sjob(m.dxbzh, wf, m.dxbzh, nullptr, 0, 0, wf);
+    L.job[n++] = sjob(m.jzh, wf, m.jzh, nullptr, 0, 0, wf);
+    L.njobs = n;
+    TRY(spec(c, L));
+    SpecLaunch M = spec_base(c, 1, SM_HC2R);
+    n = 0;
+    if (with_b) {
+        // 575-593: inverse transforms, then the mean parts of B
+        M.job[n] = sjob(m.bxh, wf, c->bx, nullptr, 0, 0, wf);
+        M.job[n++].addk = c->bxbar;
+        M.job[n] = sjob(m.byh, wf, c->by, nullptr, 0, 0, wf);
+        M.job[n++].addk = c->bybar;
+        M.job[n] = sjob(m.bzh, wf, c->bz, nullptr, 0, 0, wf);
+        M.job[n++].adds = c->sc.bzbar;
+    }
+    M.job[n++] = sjob(m.dybzh, wf, m.dybzh, nullptr, 0, 0, wf);
+    M.job[n++] = sjob(m.dxbzh, wf, m.dxbzh, nullptr, 0, 0, wf);
+    M.job[n++] = sjob(m.jzh, wf, m.jzh, nullptr, 0, 0, wf);
+    M.njobs = n;
+    TRY(spec(c, M));
+    return kerr(launch_jsq(m, c->s));
+}
+
+int run_bandj(mhd_ctx* c)
+{
+    const int wf = c->wf32;
+    MagArgs m = margs(c);
+    ZLaunch Z = zbase(c);
+    Z.job[0] = zjob(c->pol, 0, m.polz, wf, ZK_D1, 5);  // 519
+    Z.njobs = 1;
+    TRY(zrun(c, Z));
+    TRY(kerr(launch_bandj_spec(m, c->s)));
+    return inverse_b_and_j(c, m, true);
+}
+
+int run_mag_boundary(mhd_ctx* c, int hydro, int mag)
+{
+    BCArgs b;
+    memset(&b, 0, sizeof(b));
+    b.g = c->g;
+    b.theta = c->p.theta;
+    b.u = c->u; b.v = c->v; b.w = c->w; b.t = c->t;
+    b.pol = c->pol; b.tor = c->tor; b.bxbar = c->bxbar; b.bybar = c->bybar;
+    b.hydro = hydro;
+    b.mag = mag;
+    return kerr(launch_boundary(b, c->s));
+}
+
+// mag_init, src/magnetic.f90:5-279
+int run_mag_init(mhd_ctx* c)
+{
+    const int wf = c->wf32;
+    MagArgs m = margs(c);
+    SpecLaunch L = spec_base(c, 0, SM_R2HC);
+    L.job[0] = sjob(c->bx, wf, m.exh, nullptr, 0, 0, wf);
+    L.job[1] = sjob(c->by, wf, m.eyh, nullptr, 0, 0, wf);
+    L.job[2] = sjob(c->bz, wf, m.ezh, nullptr, 0, 0, wf);
+    L.njobs = 3;
+    TRY(spec(c, L));
+    SpecLaunch M = spec_base(c, 1, SM_R2HC);
+    M.job[0] = sjob(m.exh, wf, m.exh, nullptr, 0, 0, wf);
+    M.job[1] = sjob(m.eyh, wf, m.eyh, nullptr, 0, 0, wf);
+    M.job[2] = sjob(m.ezh, wf, m.ezh, nullptr, 0, 0, wf);
+    M.njobs = 3;
+    TRY(spec(c, M));
+    TRY(kerr(launch_mag_init_spec(m, c->s, c->d_bzbar)));
+    MHD_HIP_CHECK(hipMemcpyAsync(&c->sc.bzbar, c->d_bzbar, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    TRY(inverse_b_and_j(c, m, false));
+    return run_mag_boundary(c, 0, 1);
+}
+
+// 5-field dealias (src/main.F90:495-502), in place
+int run_dealias5(mhd_ctx* c)
+{
+    double* f5[5] = {c->r, c->t, c->u, c->v, c->w};
+    for (int dir = 0; dir < 2; ++dir) {
+        SpecLaunch L = spec_base(c, dir, SM_DERIV1);
+        for (int i = 0; i < 5; ++i) L.job[i] = sjob(f5[i], 0, f5[i], nullptr, MUL_MASK, 0, 0);
+        L.njobs = 5;
+        if (c->wf32) {
+            // the transform works in the compute type; the state stays FP64
+            for (int i = 0; i < 5; ++i) L.job[i].out_f32 = 0;
+        }
+        TRY(spec(c, L));
+    }
+    return 0;
+}
+
+RedArgs rargs(mhd_ctx* c, int op)
+{
+    RedArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = c->g;
+    a.p = c->ph;
+    a.op = op;
+    a.u = c->u; a.v = c->v; a.w = c->w; a.r = c->r; a.t = c->t;
+    a.bx = c->bx; a.by = c->by; a.bz = c->bz;
+    a.wf32 = c->wf32;
+    a.partial = c->red_partial;
+    a.result = c->red_result;
+    return a;
+}
+
+int fetch_result(mhd_ctx* c, double* out, int n)
+{
+    MHD_HIP_CHECK(hipMemcpyAsync(c->hbuf, c->red_result, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    memcpy(out, c->hbuf, n * sizeof(double));
+    return 0;
+}
+
+int run_global_averages(mhd_ctx* c, double out[4])
+{
+    RedArgs a = rargs(c, RED_GLOBALAVG);
+    TRY(kerr(launch_reduce(a, c->s)));
+    TRY(kerr(launch_com_shift(c->g, c->u, c->v, c->red_result, c->s)));
+    return out ? fetch_result(c, out, 4) : 0;
+}
+
+int run_nusselt(mhd_ctx* c, double* nu)
+{
+    RedArgs a = rargs(c, RED_NUSSELT);
+    TRY(kerr(launch_reduce(a, c->s)));
+    return fetch_result(c, nu, 1);
+}
+
+int run_checkdivb(mhd_ctx* c, double out[2])
+{
+    // src/utils.F90:262-264: d/dx bx + d/dy by + d/dz bz (flag 5)
+    const int wf = c->wf32;
+    void** W = c->wk;
+    SpecLaunch L = spec_base(c, 0, SM_DERIV1);
+    L.job[0] = sjob(c->bx, wf, W[W_UY], nullptr, MUL_D1, 0, wf);
+    L.njobs = 1;
+    TRY(spec(c, L));
+    SpecLaunch M = spec_base(c, 1, SM_DERIV1);
+    M.job[0] = sjob(c->by, wf, W[W_UYY], nullptr, MUL_D1, 0, wf);
+    M.njobs = 1;
+    TRY(spec(c, M));
+    ZLaunch Z = zbase(c);
+    Z.job[0] = zjob(c->bz, wf, W[W_VY], wf, ZK_D1, 5);
+    Z.njobs = 1;
+    TRY(zrun(c, Z));
+    RedArgs a = rargs(c, RED_DIVB);
+    a.f0 = W[W_UY]; a.f1 = W[W_UYY]; a.f2 = W[W_VY];
+    a.f_f32 = wf;
+    TRY(kerr(launch_reduce(a, c->s)));
+    return fetch_result(c, out, 2);
+}
+
+int run_calcdt(mhd_ctx* c, double* dt)
+{
+    RedArgs a = rargs(c, RED_CALCDT);
+    TRY(kerr(launch_reduce(a, c->s)));
+    return fetch_result(c, dt, 1);
+}
+
+int check_ctx(mhd_ctx* c) { return c ? 0 : MHD_E_ARG; }
+
+// AB3 coefficients with the Euler / AB2 start-up, src/main.F90:381-403
+void ab3_coeffs(mhd_scalars_t& s, double dt, double cf[3])
+{
+    if (s.first) {
+        s.first = 0;
+        s.second = 1;
+        s.dt1 = 0.0;
+        s.dt2 = 0.0;
+        cf[0] = dt; cf[1] = 0.0; cf[2] = 0.0;
+    } else if (s.second) {
+        s.first = 0;
+        s.second = 0;
+        s.dt2 = 0.0;
+        cf[0] = dt * (1.0 + 0.5 * (dt / s.dt1));
+        cf[1] = -0.5 * dt * (dt / s.dt1);
+        cf[2] = 0.0;
+    } else {
+        const double dt1 = s.dt1, dt2 = s.dt2;
+        cf[0] = dt + (((dt * dt) / (dt1 * dt2)) * ((dt / 3.0) + 0.5 * (dt1 + dt2)));
+        cf[1] = ((dt * dt) / (dt1 * (dt1 - dt2))) * ((dt / 3.0) + 0.5 * dt2);
+        cf[2] = -((dt * dt) / (dt2 * (dt1 - dt2))) * ((dt / 3.0) + 0.5 * dt1);
+    }
+}
+
+int run_ab3(mhd_ctx* c, const double cf[3])
+{
+    AB3Args a;
+    memset(&a, 0, sizeof(a));
+    a.g = c->g;
+    a.c0 = cf[0]; a.c1 = cf[1]; a.c2 = cf[2];
+    double* vars[7] = {c->u, c->v, c->w, c->r, c->t, c->pol, c->tor};
+    for (int f = 0; f < 7; ++f) {
+        a.var[f] = vars[f];
+        for (int l = 0; l < 3; ++l) a.hist[f][l] = level_ptr(c, f, l + 1);
+    }
+    a.mvar[0] = c->bxbar;
+    a.mvar[1] = c->bybar;
+    for (int f = 0; f < 2; ++f)
+        for (int l = 0; l < 3; ++l) a.mhist[f][l] = mlevel_ptr(c, f, l + 1);
+    a.mag = c->ph.mag;
+    a.nfields = a.mag ? 7 : 5;
+    a.icutx = c->icutx;
+    a.icuty = c->icuty;
+    TRY(timed(c, MHD_K_AB3, [&] { return kerr(launch_ab3(a, c->s)); }));
+    // shift 2->1, 3->2 (src/main.F90:448-476): relabel the ring
+    const int r0 = c->ring[0];
+    c->ring[0] = c->ring[1];
+    c->ring[1] = c->ring[2];
+    c->ring[2] = r0;
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void mhd_ra_q_to_ck_f(double gamma, double sigma, double theta, double cm, double zeta, double ra, double q,
+                      double* ck, double* f)
+{
+    // src/initial.F90:187-193
+    const double c = std::sqrt((((cm + 1.0) * theta * theta) / (sigma * ra * gamma)) * (cm + 1.0 - (cm * gamma)) *
+                               std::pow(1.0 + (0.5 * theta), (2.0 * cm) - 1.0));
+    *ck = c;
+    *f = q * (zeta * sigma * c * c);
+}
+
+const char* mhd_strerror(int code)
+{
+    switch (code) {
+    case MHD_OK: return "ok";
+    case MHD_E_ODD_NX: return "nx must be even";
+    case MHD_E_ODD_NY: return "ny must be even";
+    case MHD_E_ALLOC: return "allocation failed";
+    case MHD_E_NZ: return "need at least six layers";
+    case MHD_E_FLAG: return "unknown derivative flag";
+    case MHD_E_DIVB: return "div B is not zero";
+    case MHD_E_ARG: return "bad argument";
+    case MHD_E_SIZE: return "horizontal size has no transform kernel";
+    case MHD_E_DEVICE: return "HIP runtime error";
+    case MHD_E_STATE: return "call out of order";
+    }
+    return "unknown error";
+}
+
+int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd_ctx_t** out)
+{
+    if (!p || !out) return MHD_E_ARG;
+    *out = nullptr;
+    if (2 * (nx / 2) != nx) return MHD_E_ODD_NX;
+    if (2 * (ny / 2) != ny) return MHD_E_ODD_NY;
+    if (nz < 6) return MHD_E_NZ;
+    if (!spectral_len_supported(nx) || !spectral_len_supported(ny)) return MHD_E_SIZE;
+    if (precision != MHD_PREC_FP64) return MHD_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MHD_E_DEVICE;
+
+    mhd_ctx* c = new mhd_ctx();
+    memset((void*)c, 0, sizeof(*c));
+    c->nx = nx; c->ny = ny; c->nz = nz;
+    c->nxy = nx * ny;
+    c->n3 = (long)nx * ny * nz;
+    c->prec = precision;
+    c->wf32 = 0;
+    c->wsz = c->wf32 ? sizeof(float) : sizeof(double);
+    c->p = *p;
+    c->g.nx = nx; c->g.ny = ny; c->g.nz = nz; c->g.nxy = nx * ny; c->g.n3 = c->n3;
+    c->icutx = (nx / 2) / 3;
+    c->icuty = (ny / 2) / 3;
+    set_constants(c);
+    c->ring[0] = 0; c->ring[1] = 1; c->ring[2] = 2;
+    c->sc.first = 1;
+
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return MHD_E_DEVICE; }
+
+    // pool layout
+    const size_t f64 = al256(c->n3 * sizeof(double));
+    const size_t fw = al256(c->n3 * c->wsz);
+    const size_t vz = al256(nz * sizeof(double));
+    size_t bytes = 0;
+    bytes += 2 * al256(nx * sizeof(double)) + 2 * al256(ny * sizeof(double)) + 2 * vz;   // tables
+    bytes += 2 * al256(2 * nx * sizeof(double)) + 2 * al256(2 * ny * sizeof(double));   // twiddles
+    bytes += 7 * f64 + 2 * vz;                       // state
+    bytes += f64;                                    // FP64 staging for host transfers
+    bytes += 4 * fw;                                  // bx by bz jsq
+    bytes += 21 * f64 + 6 * vz;                       // history
+    bytes += W_COUNT * fw;                            // work
+    bytes += al256(nz * 8 * sizeof(double)) + 2 * al256(8 * sizeof(double));  // reductions
+    if (hipMalloc((void**)&c->pool, bytes) != hipSuccess) {
+        (void)hipStreamDestroy(c->s);
+        delete c;
+        return MHD_E_ALLOC;
+    }
+    c->pool_bytes = bytes;
+    if (hipMemsetAsync(c->pool, 0, bytes, c->s) != hipSuccess) {
+        mhd_destroy(c);
+        return MHD_E_DEVICE;
+    }
+    char* cur = c->pool;
+    auto take = [&](size_t b) { char* q = cur; cur += b; return q; };
+    c->akx1 = (double*)take(al256(nx * sizeof(double)));
+    c->akx2 = (double*)take(al256(nx * sizeof(double)));
+    c->aky1 = (double*)take(al256(ny * sizeof(double)));
+    c->aky2 = (double*)take(al256(ny * sizeof(double)));
+    c->stage = (double*)take(f64);
+    c->kappa = (double*)take(vz);
+    c->dkappa = (double*)take(vz);
+    void* twx64 = take(al256(2 * nx * sizeof(double)));
+    void* twx32 = take(al256(2 * nx * sizeof(double)));
+    void* twy64 = take(al256(2 * ny * sizeof(double)));
+    void* twy32 = take(al256(2 * ny * sizeof(double)));
+    c->twx = c->wf32 ? twx32 : twx64;
+    c->twy = c->wf32 ? twy32 : twy64;
+    c->u = (double*)take(f64); c->v = (double*)take(f64); c->w = (double*)take(f64);
+    c->t = (double*)take(f64); c->r = (double*)take(f64);
+    c->pol = (double*)take(f64); c->tor = (double*)take(f64);
+    c->bxbar = (double*)take(vz); c->bybar = (double*)take(vz);
+    c->bx = take(fw); c->by = take(fw); c->bz = take(fw); c->jsq = take(fw);
+    for (int f = 0; f < 7; ++f)
+        for (int l = 0; l < 3; ++l) c->hist[f][l] = (double*)take(f64);
+    for (int f = 0; f < 2; ++f)
+        for (int l = 0; l < 3; ++l) c->mhist[f][l] = (double*)take(vz);
+    for (int i = 0; i < W_COUNT; ++i) c->wk[i] = take(fw);
+    c->red_partial = (double*)take(al256(nz * 8 * sizeof(double)));
+    c->red_result = (double*)take(al256(8 * sizeof(double)));
+    c->d_bzbar = (double*)take(al256(8 * sizeof(double)));
+    if ((size_t)(cur - c->pool) > bytes) {
+        fprintf(stderr, "mhd: pool overflow\n");
+        mhd_destroy(c);
+        return MHD_E_ALLOC;
+    }
+    if (hipHostMalloc((void**)&c->hbuf, (size_t)c->n3 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        mhd_destroy(c);
+        return MHD_E_ALLOC;
+    }
+
+    // tables
+    std::vector<double> a1, a2, b1, b2, ka, dka, tw64;
+    std::vector<float> tw32;
+    wavenumbers(nx, p->xmax, a1, a2);
+    wavenumbers(ny, p->ymax, b1, b2);
+    kappa_profile(nz, c->dz, ka, dka);
+    int e = 0;
+    e |= hipMemcpy(c->akx1, a1.data(), nx * sizeof(double), hipMemcpyHostToDevice);
+    e |= hipMemcpy(c->akx2, a2.data(), nx * sizeof(double), hipMemcpyHostToDevice);
+    e |= hipMemcpy(c->aky1, b1.data(), ny * sizeof(double), hipMemcpyHostToDevice);
+    e |= hipMemcpy(c->aky2, b2.data(), ny * sizeof(double), hipMemcpyHostToDevice);
+    e |= hipMemcpy(c->kappa, ka.data(), nz * sizeof(double), hipMemcpyHostToDevice);
+    e |= hipMemcpy(c->dkappa, dka.data(), nz * sizeof(double), hipMemcpyHostToDevice);
+    twiddles(nx, tw64, tw32);
+    e |= hipMemcpy(twx64, tw64.data(), 2 * nx * sizeof(double), hipMemcpyHostToDevice);
+    e |= hipMemcpy(twx32, tw32.data(), 2 * nx * sizeof(float), hipMemcpyHostToDevice);
+    twiddles(ny, tw64, tw32);
+    e |= hipMemcpy(twy64, tw64.data(), 2 * ny * sizeof(double), hipMemcpyHostToDevice);
+    e |= hipMemcpy(twy32, tw32.data(), 2 * ny * sizeof(float), hipMemcpyHostToDevice);
+    if (e || hipDeviceSynchronize() != hipSuccess) {
+        mhd_destroy(c);
+        return MHD_E_DEVICE;
+    }
+    for (int i = 0; i < 2 * MAXEV; ++i)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            mhd_destroy(c);
+            return MHD_E_DEVICE;
+        } else
+            c->ev_created = i + 1;
+    *out = c;
+    return MHD_OK;
+}
+
+void mhd_destroy(mhd_ctx_t* c)
+{
+    if (!c) return;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    for (int i = 0; i < c->ev_created; ++i) (void)hipEventDestroy(c->ev[i]);
+    if (c->hbuf) (void)hipHostFree(c->hbuf);
+    if (c->pool) (void)hipFree(c->pool);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+}
+
+static int upload(mhd_ctx* c, void* dst, int dst_f32, const double* src, long n)
+{
+    if (!src) return 0;
+    memcpy(c->hbuf, src, n * sizeof(double));
+    if (!dst_f32) {
+        MHD_HIP_CHECK(hipMemcpyAsync(dst, c->hbuf, n * sizeof(double), hipMemcpyHostToDevice, c->s));
+    } else {
+        MHD_HIP_CHECK(hipMemcpyAsync(c->stage, c->hbuf, n * sizeof(double), hipMemcpyHostToDevice, c->s));
+        TRY(kerr(launch_copy_cvt(c->stage, 0, dst, 1, n, c->s)));
+    }
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+static int download(mhd_ctx* c, double* dst, const void* src, int src_f32, long n)
+{
+    if (!dst) return 0;
+    const void* from = src;
+    if (src_f32) {
+        TRY(kerr(launch_copy_cvt(src, 1, c->stage, 0, n, c->s)));
+        from = c->stage;
+    }
+    MHD_HIP_CHECK(hipMemcpyAsync(c->hbuf, from, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    memcpy(dst, c->hbuf, n * sizeof(double));
+    return 0;
+}
+
+int mhd_static(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    // src/initial.F90:25-86 on the host (integer RNG), then upload
+    const int nz = c->nz, m = c->nxy;
+    const long n3 = c->n3;
+    std::vector<double> t(n3), r(n3), zero(n3, 0.0), one(n3, 1.0), bz(n3, 1.0);
+    for (int k = 0; k < nz; ++k) {
+        const double z = (double)k * c->dz;
+        const double t0 = 1.0 + z * c->p.theta;
+        const double r0 = std::pow(t0, c->p.cm);
+        int iseed = 1996 + 64 * (k + 1);
+        for (int q = 0; q < m; ++q) t[(long)k * m + q] = 2.0 * (ran0(&iseed) - 0.5) * c->p.t_amp;
+        for (int q = 0; q < m; ++q) {
+            r[(long)k * m + q] = r0;
+            t[(long)k * m + q] = t[(long)k * m + q] + t0;
+        }
+        if (c->p.b_amp != 0.0) {
+            int bseed = 1996 + 64 * (k + 1) + 32;
+            for (int q = 0; q < m; ++q) bz[(long)k * m + q] += 2.0 * (ran0(&bseed) - 0.5) * c->p.b_amp;
+        }
+    }
+    return mhd_set_state(c, r.data(), t.data(), zero.data(), zero.data(), zero.data(), zero.data(), zero.data(),
+                         bz.data());
+}
+
+int mhd_set_state(mhd_ctx_t* c, const double* r, const double* t, const double* u, const double* v, const double* w,
+                  const double* bx, const double* by, const double* bz)
+{
+    TRY(check_ctx(c));
+    if (!r || !t || !u || !v || !w || !bx || !by || !bz) return MHD_E_ARG;
+    const long n = c->n3;
+    TRY(upload(c, c->r, 0, r, n));
+    TRY(upload(c, c->t, 0, t, n));
+    TRY(upload(c, c->u, 0, u, n));
+    TRY(upload(c, c->v, 0, v, n));
+    TRY(upload(c, c->w, 0, w, n));
+    TRY(upload(c, c->bx, c->wf32, bx, n));
+    TRY(upload(c, c->by, c->wf32, by, n));
+    TRY(upload(c, c->bz, c->wf32, bz, n));
+    if (c->ph.mag) TRY(run_mag_init(c));
+    c->have_state = 1;
+    return mhd_sync(c);
+}
+
+int mhd_get_state(mhd_ctx_t* c, double* r, double* t, double* u, double* v, double* w, double* bx, double* by,
+                  double* bz)
+{
+    TRY(check_ctx(c));
+    const long n = c->n3;
+    TRY(download(c, r, c->r, 0, n));
+    TRY(download(c, t, c->t, 0, n));
+    TRY(download(c, u, c->u, 0, n));
+    TRY(download(c, v, c->v, 0, n));
+    TRY(download(c, w, c->w, 0, n));
+    TRY(download(c, bx, c->bx, c->wf32, n));
+    TRY(download(c, by, c->by, c->wf32, n));
+    TRY(download(c, bz, c->bz, c->wf32, n));
+    return 0;
+}
+
+static int field_ref(mhd_ctx* c, int id, int level, void** ptr, int* f32, long* n)
+{
+    *f32 = 0;
+    *n = c->n3;
+    switch (id) {
+    case MHD_F_U: *ptr = c->u; return 0;
+    case MHD_F_V: *ptr = c->v; return 0;
+    case MHD_F_W: *ptr = c->w; return 0;
+    case MHD_F_T: *ptr = c->t; return 0;
+    case MHD_F_R: *ptr = c->r; return 0;
+    case MHD_F_POL: *ptr = c->pol; return 0;
+    case MHD_F_TOR: *ptr = c->tor; return 0;
+    case MHD_F_BX: *ptr = c->bx; *f32 = c->wf32; return 0;
+    case MHD_F_BY: *ptr = c->by; *f32 = c->wf32; return 0;
+    case MHD_F_BZ: *ptr = c->bz; *f32 = c->wf32; return 0;
+    case MHD_F_JSQ: *ptr = c->jsq; *f32 = c->wf32; return 0;
+    case MHD_F_BXBAR: *ptr = c->bxbar; *n = c->nz; return 0;
+    case MHD_F_BYBAR: *ptr = c->bybar; *n = c->nz; return 0;
+    case MHD_F_KAPPA: *ptr = c->kappa; *n = c->nz; return 0;
+    case MHD_F_DKAPPA: *ptr = c->dkappa; *n = c->nz; return 0;
+    }
+    if (level < 1 || level > 3) return MHD_E_ARG;
+    if (id == MHD_F_BXBARDOT || id == MHD_F_BYBARDOT) {
+        *ptr = mlevel_ptr(c, id == MHD_F_BXBARDOT ? 0 : 1, level);
+        *n = c->nz;
+        return 0;
+    }
+    const int h = hist_index(id);
+    if (h < 0) return MHD_E_ARG;
+    *ptr = level_ptr(c, h, level);
+    return 0;
+}
+
+int mhd_get_field(mhd_ctx_t* c, int id, int level, double* dst)
+{
+    TRY(check_ctx(c));
+    if (!dst) return MHD_E_ARG;
+    void* p;
+    int f32;
+    long n;
+    TRY(field_ref(c, id, level, &p, &f32, &n));
+    return download(c, dst, p, f32, n);
+}
+
+int mhd_set_field(mhd_ctx_t* c, int id, int level, const double* src)
+{
+    TRY(check_ctx(c));
+    if (!src) return MHD_E_ARG;
+    void* p;
+    int f32;
+    long n;
+    TRY(field_ref(c, id, level, &p, &f32, &n));
+    return upload(c, p, f32, src, n);
+}
+
+int mhd_get_scalars(mhd_ctx_t* c, mhd_scalars_t* s)
+{
+    TRY(check_ctx(c));
+    if (!s) return MHD_E_ARG;
+    *s = c->sc;
+    return 0;
+}
+
+int mhd_set_scalars(mhd_ctx_t* c, const mhd_scalars_t* s)
+{
+    TRY(check_ctx(c));
+    if (!s) return MHD_E_ARG;
+    c->sc = *s;
+    return 0;
+}
+
+int mhd_hydro_deriv(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    TRY(run_hydro_deriv(c));
+    return mhd_sync(c);
+}
+
+int mhd_mag_deriv(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    TRY(run_mag_deriv(c));
+    return mhd_sync(c);
+}
+
+int mhd_bandj(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    TRY(run_bandj(c));
+    return mhd_sync(c);
+}
+
+int mhd_magdealias(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    TRY(kerr(launch_magdealias(c->g, c->pol, c->tor, c->icutx, c->icuty, c->s)));
+    return mhd_sync(c);
+}
+
+int mhd_boundary(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    TRY(run_mag_boundary(c, 1, 0));
+    return mhd_sync(c);
+}
+
+int mhd_mag_boundary(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    TRY(run_mag_boundary(c, 0, 1));
+    return mhd_sync(c);
+}
+
+int mhd_mag_init(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    TRY(run_mag_init(c));
+    return mhd_sync(c);
+}
+
+int mhd_step(mhd_ctx_t* c, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* last)
+{
+    TRY(check_ctx(c));
+    if (!c->have_state) return MHD_E_STATE;
+    const bool mag = c->ph.mag;
+    if (dt_fixed > 0.0) c->sc.dt = dt_fixed;
+    else if (c->sc.dt <= 0.0) TRY(run_calcdt(c, &c->sc.dt));   // src/main.F90:318
+    for (int s = 0; s < nsteps; ++s) {
+        mhd_scalars_t& S = c->sc;
+        S.nit += 1;
+        S.iter += 1;
+        S.ttime += S.dt;
+        TRY(run_hydro_deriv(c));
+        if (mag) TRY(run_mag_deriv(c));
+        double cf[3];
+        const double dt = S.dt;
+        ab3_coeffs(S, dt, cf);
+        TRY(run_ab3(c, cf));   // includes magdealias (src/main.F90:508)
+        S.dt2 = dt + S.dt1;
+        S.dt1 = dt;
+        if (S.nit % 10 == 0) TRY(run_dealias5(c));
+        if (mag) TRY(run_bandj(c));
+        TRY(run_mag_boundary(c, 1, mag ? 1 : 0));
+        if (ndu06 > 0 && S.nit % ndu06 == 0) {
+            double ga[4], nu, db[2];
+            TRY(run_global_averages(c, ga));
+            TRY(run_nusselt(c, &nu));
+            TRY(run_checkdivb(c, db));
+            if (last) {
+                last->ubar = ga[0];
+                last->vbar = ga[1];
+                last->kinenergy = ga[2];
+                last->magenergy = mag ? c->p.f * ga[3] : ga[3];
+                last->nusselt = nu;
+                last->divbmax = db[0];
+                last->divbmin = db[1];
+                last->dt = S.dt;
+                last->nit = S.nit;
+            }
+            if (std::fmax(std::fabs(db[0]), std::fabs(db[1])) > 0.01) return MHD_E_DIVB;
+        }
+        if (dt_fixed <= 0.0 && S.nit % 10 == 0) TRY(run_calcdt(c, &S.dt));
+    }
+    return 0;
+}
+
+int mhd_global_averages(mhd_ctx_t* c, double out[4])
+{
+    TRY(check_ctx(c));
+    return run_global_averages(c, out);
+}
+
+int mhd_nusselt(mhd_ctx_t* c, double* nu)
+{
+    TRY(check_ctx(c));
+    return run_nusselt(c, nu);
+}
+
+int mhd_checkdivb(mhd_ctx_t* c, double out[2])
+{
+    TRY(check_ctx(c));
+    return run_checkdivb(c, out);
+}
+
+int mhd_sminmax(mhd_ctx_t* c, int id, double out[2])
+{
+    TRY(check_ctx(c));
+    void* p;
+    int f32;
+    long n;
+    TRY(field_ref(c, id, 1, &p, &f32, &n));
+    if (n != c->n3) return MHD_E_ARG;
+    RedArgs a = rargs(c, RED_MINMAX);
+    a.f0 = p;
+    a.f_f32 = f32;
+    TRY(kerr(launch_reduce(a, c->s)));
+    return fetch_result(c, out, 2);
+}
+
+int mhd_calcdt(mhd_ctx_t* c, double* dt)
+{
+    TRY(check_ctx(c));
+    return run_calcdt(c, dt);
+}
+
+int mhd_op(mhd_ctx_t* c, int op, const double* in, double* out, double* out2, int flag)
+{
+    TRY(check_ctx(c));
+    if (!in || !out) return MHD_E_ARG;
+    const int wf = c->wf32;
+    const long n3 = c->n3;
+    void* A = c->wk[W_C1];
+    void* B = c->wk[W_C2];
+    void* C = c->wk[W_C3];
+    if (op == MHD_OP_D1BYDZ1D || op == MHD_OP_D2BYDZ1D) {
+        // nz-vectors: run the 3-D kernel on a 1x1 column broadcast over the plane
+        std::vector<double> col(n3);
+        for (int k = 0; k < c->nz; ++k)
+            for (int q = 0; q < c->nxy; ++q) col[(long)k * c->nxy + q] = in[k];
+        TRY(upload(c, A, wf, col.data(), n3));
+        ZLaunch Z = zbase(c);
+        Z.job[0] = zjob(A, wf, B, wf, op == MHD_OP_D1BYDZ1D ? ZK_D1 : ZK_D2, flag);
+        Z.njobs = 1;
+        TRY(zrun(c, Z));
+        TRY(download(c, col.data(), B, wf, n3));
+        for (int k = 0; k < c->nz; ++k) out[k] = col[(long)k * c->nxy];
+        return 0;
+    }
+    TRY(upload(c, A, wf, in, n3));
+    switch (op) {
+    case MHD_OP_D1BYDX:
+    case MHD_OP_D1BYDY: {
+        SpecLaunch L = spec_base(c, op == MHD_OP_D1BYDX ? 0 : 1, SM_DERIV1);
+        L.job[0] = sjob(A, wf, B, nullptr, MUL_D1, 0, wf);
+        L.njobs = 1;
+        TRY(spec(c, L));
+        return download(c, out, B, wf, n3);
+    }
+    case MHD_OP_D2BYDX:
+    case MHD_OP_D2BYDY: {
+        if (!out2) return MHD_E_ARG;
+        SpecLaunch L = spec_base(c, op == MHD_OP_D2BYDX ? 0 : 1, SM_DERIV2);
+        L.job[0] = sjob(A, wf, B, C, MUL_D1, MUL_D2, wf);
+        L.njobs = 1;
+        TRY(spec(c, L));
+        TRY(download(c, out, B, wf, n3));
+        return download(c, out2, C, wf, n3);
+    }
+    case MHD_OP_DEALIAS: {
+        for (int dir = 0; dir < 2; ++dir) {
+            SpecLaunch L = spec_base(c, dir, SM_DERIV1);
+            L.job[0] = sjob(A, wf, A, nullptr, MUL_MASK, 0, wf);
+            L.njobs = 1;
+            TRY(spec(c, L));
+        }
+        return download(c, out, A, wf, n3);
+    }
+    case MHD_OP_FORWARDFFT:
+    case MHD_OP_INVERSEFFT: {
+        const int mode = op == MHD_OP_FORWARDFFT ? SM_R2HC : SM_HC2R;
+        for (int dir = 0; dir < 2; ++dir) {
+            SpecLaunch L = spec_base(c, dir, mode);
+            L.job[0] = sjob(A, wf, A, nullptr, 0, 0, wf);
+            L.njobs = 1;
+            TRY(spec(c, L));
+        }
+        return download(c, out, A, wf, n3);
+    }
+    case MHD_OP_D1BYDZ:
+    case MHD_OP_D1UPBYDZ:
+    case MHD_OP_D2BYDZ: {
+        ZLaunch Z = zbase(c);
+        const int kind = op == MHD_OP_D1BYDZ ? ZK_D1 : op == MHD_OP_D2BYDZ ? ZK_D2 : ZK_D1UP;
+        Z.job[0] = zjob(A, wf, B, wf, kind, flag);
+        Z.njobs = 1;
+        TRY(zrun(c, Z));
+        return download(c, out, B, wf, n3);
+    }
+    }
+    return MHD_E_ARG;
+}
+
+int mhd_kernel_timing(mhd_ctx_t* c, int kernel, int enable)
+{
+    TRY(check_ctx(c));
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    c->nev = 0;
+    c->timed_kernel = enable ? kernel : MHD_K_NONE;
+    return 0;
+}
+
+int mhd_kernel_time(mhd_ctx_t* c, double* total_ms, long* launches)
+{
+    TRY(check_ctx(c));
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    double tot = 0.0;
+    for (int i = 0; i + 1 < c->nev; i += 2) {
+        float ms = 0.f;
+        MHD_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = c->nev / 2;
+    return 0;
+}
+
+int mhd_sync(mhd_ctx_t* c)
+{
+    TRY(check_ctx(c));
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    MHD_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+long mhd_device_bytes(mhd_ctx_t* c) { return c ? (long)c->pool_bytes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,780 @@
+// physics.hip -- vertical derivatives, right-hand-side assembly, magnetic
+// spectral operations, AB3 update, boundary conditions and reductions for
+// gfx950.  Point-wise kernels use one thread per grid point with x fastest,
+// so every wavefront reads 64 consecutive elements of each operand.
+#include "physics.hpp"
+#include "spectral.hpp"
+#include "zstencil.hpp"
+
+namespace mhd {
+
+static constexpr int NT = 256;
+
+static inline unsigned nblk(long n, int nt = NT) { return (unsigned)((n + nt - 1) / nt); }
+
+// ---------------------------------------------------------------------------
+// batched z-derivatives
+// ---------------------------------------------------------------------------
+template <typename TC>
+__global__ void __launch_bounds__(NT) z_kernel(ZLaunch L)
+{
+    const ZJob& J = L.job[blockIdx.y];
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= L.g.n3) return;
+    const int nxy = L.g.nxy, nz = L.g.nz;
+    const int k = (int)(q / nxy);
+    const long col = q - (long)k * nxy;
+    auto F = [&](int kk) -> TC {
+        const long i = col + (long)kk * nxy;
+        if (J.load_op == LD_PROD)
+            return ((TC)J.scale * ldv<TC>(J.in, i, J.in_f32)) * ldv<TC>(J.in1, i, J.in_f32);
+        return ldv<TC>(J.in, i, J.in_f32);
+    };
+    TC d;
+    if (J.kind == ZK_D1) d = dz1<TC>(k, nz, J.flag, (TC)L.ho12, F);
+    else if (J.kind == ZK_D2) d = dz2<TC>(k, nz, J.flag, (TC)L.sc1, F);
+    else d = dz1up<TC>(k, nz, J.flag, (TC)L.ho12, ldv<TC>(L.w, q, L.w_f32), F);
+    stv<TC>(J.out, q, J.out_f32, d);
+}
+
+int launch_z(const ZLaunch& L, hipStream_t s)
+{
+    if (L.njobs <= 0) return 0;
+    for (int i = 0; i < L.njobs; ++i)
+        if (!zflag_ok(L.job[i].kind, L.job[i].flag)) return 7;
+    dim3 grid(nblk(L.g.n3), L.njobs);
+    bool f32 = L.job[0].out_f32 != 0;
+    if (f32) hipLaunchKernelGGL(z_kernel<float>, grid, dim3(NT), 0, s, L);
+    else hipLaunchKernelGGL(z_kernel<double>, grid, dim3(NT), 0, s, L);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+// ---------------------------------------------------------------------------
+// hydro RHS, src/hydro_deriv.f90.  Stage A holds every term that needs only
+// primitive-field derivatives, in the reference's order of accumulation
+// (lines 45-432) except the three composite-flux derivatives, which stage B
+// adds once they exist, followed by the boundary d(rho)/dz substitution
+// (455-482) and the division by rho (487-497).
+// ---------------------------------------------------------------------------
+template <typename TW>
+__global__ void __launch_bounds__(NT) hydro_a_kernel(HydroArgs a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= a.g.n3) return;
+    const int k = (int)(q / a.g.nxy);
+    const Phys& p = a.p;
+    auto W = [&](const void* ptr) -> double { return (double)((const TW*)ptr)[q]; };
+    const double rr = a.r[q], tt = a.t[q], u0 = a.u[q], v0 = a.v[q], w0 = a.w[q];
+    const double kap = a.kappa[k], dkap = a.dkappa[k];
+    // 45-53, 82-110
+    const double rx = W(a.rx), ry = W(a.ry), tx = W(a.tx), ty = W(a.ty), tzu = W(a.tzu);
+    double rdot = -u0 * rx - v0 * ry;
+    const double exv = -tt * rx, eyv = -tt * ry;
+    double tdot = -rr * (u0 * tx + v0 * ty + w0 * tzu) + p.gck * kap * (W(a.txx) + W(a.tyy) + W(a.tzz)) +
+                  p.gck * dkap * tzu + p.fzckg * W(a.jsq);
+    double udot = exv - rr * tx;
+    double vdot = eyv - rr * ty;
+    double wdot = -rr * W(a.tzc);
+    // 139-168
+    const double z1 = W(a.ux), z2 = W(a.vy), z3 = W(a.wzu), z4 = W(a.wzc);
+    double divu = z1 + z2 + z4;
+    rdot = rdot - rr * divu;
+    tdot = tdot - p.gm1 * tt * divu * rr + 2 * p.sckg * (z1 * z1 + z2 * z2 + z4 * z4 - (1.0 / 3.0) * divu * divu);
+    const double bxv = W(a.bx), byv = W(a.by), bzv = W(a.bz);
+    const double temp1 = p.fo2 * bxv * bxv, temp2 = p.fo2 * byv * byv, temp3 = p.fo2 * bzv * bzv;
+    divu = p.scko3 * divu;
+    udot = udot - rr * u0 * z1;
+    vdot = vdot - rr * v0 * z2;
+    wdot = wdot - rr * w0 * z3;
+    ((TW*)a.c1)[q] = (TW)(divu + temp1 - temp2 - temp3);
+    ((TW*)a.c2)[q] = (TW)(divu + temp2 - temp1 - temp3);
+    ((TW*)a.c3)[q] = (TW)(p.scko3 * z1 + p.scko3 * z2 + temp3 - temp1 - temp2);
+    ((TW*)a.c4)[q] = (TW)(4.0 * p.scko3 * z4);
+    // 201-219 (the d/dx, d/dy, d/dz of c1..c4 follow in stage B)
+    {
+        const double w1 = W(a.uy), w2 = W(a.vx);
+        udot = udot - rr * v0 * w1;
+        vdot = vdot - rr * u0 * w2;
+        tdot = tdot + p.sckg * (w1 + w2) * (w1 + w2);
+    }
+    // 245-261
+    udot = udot + W(a.ybb);
+    vdot = vdot + W(a.xbb);
+    {
+        const double w3 = W(a.wx), w4 = W(a.uzu);
+        udot = udot - rr * w0 * w4;
+        wdot = wdot - rr * u0 * w3;
+        tdot = tdot + p.sckg * (w3 + w4) * (w3 + w4);
+    }
+    // 287-309
+    udot = udot + W(a.zzx);
+    wdot = wdot + W(a.xzx) + p.thmplus1 * rr;
+    {
+        const double w1 = W(a.wy), w2 = W(a.vzu);
+        vdot = vdot - rr * w0 * w2;
+        wdot = wdot - rr * v0 * w1;
+        tdot = tdot + p.sckg * (w1 + w2) * (w1 + w2);
+    }
+    udot = udot - rr * p.cor * (w0 * p.cpsi + v0 * p.spsi);
+    vdot = vdot - rr * p.cor * (-u0 * p.spsi);
+    wdot = wdot - rr * p.cor * (-u0 * p.cpsi);
+    // 331-338
+    vdot = vdot + W(a.zzy);
+    wdot = wdot + W(a.yzy);
+    // 360-432
+    udot = udot + p.sck * W(a.uzz);
+    vdot = vdot + p.sck * W(a.vzz);
+    udot = udot + p.sck * W(a.uxx);
+    vdot = vdot + p.sck * W(a.vxx);
+    wdot = wdot + p.sck * W(a.wxx);
+    udot = udot + p.sck * W(a.uyy);
+    vdot = vdot + p.sck * W(a.vyy);
+    wdot = wdot + p.sck * W(a.wyy);
+    a.rdot[q] = rdot;
+    a.tdot[q] = tdot;
+    a.udot[q] = udot;
+    a.vdot[q] = vdot;
+    a.wdot[q] = wdot;
+}
+
+template <typename TW>
+__global__ void __launch_bounds__(NT) hydro_b_kernel(HydroArgs a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= a.g.n3) return;
+    const int k = (int)(q / a.g.nxy);
+    auto W = [&](const void* ptr) -> double { return (double)((const TW*)ptr)[q]; };
+    double udot = a.udot[q] + W(a.dxc1);
+    double vdot = a.vdot[q] + W(a.dyc2);
+    double wdot = a.wdot[q] + W(a.dzc3) + W(a.dzc4);
+    double rdot = a.rdot[q], tdot = a.tdot[q];
+    const double tt = a.t[q];
+    double rzu = W(a.rzu), rzc = W(a.rzc);
+    if (k == 0 || k == a.g.nz - 1) {
+        rzu = wdot / tt;
+        rzc = wdot / tt;
+    }
+    rdot = rdot - a.w[q] * rzu;
+    wdot = wdot - tt * rzc;
+    const double ri = 1.0 / (a.r[q]);
+    a.rdot[q] = rdot;
+    a.tdot[q] = tdot * ri;
+    a.udot[q] = udot * ri;
+    a.vdot[q] = vdot * ri;
+    a.wdot[q] = wdot * ri;
+}
+
+int launch_hydro_a(const HydroArgs& a, hipStream_t s)
+{
+    if (a.wf32) hipLaunchKernelGGL(hydro_a_kernel<float>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL(hydro_a_kernel<double>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+int launch_hydro_b(const HydroArgs& a, hipStream_t s)
+{
+    if (a.wf32) hipLaunchKernelGGL(hydro_b_kernel<float>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL(hydro_b_kernel<double>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+// ---------------------------------------------------------------------------
+// magnetic part on the double-halfcomplex layout (src/magnetic.f90)
+// ---------------------------------------------------------------------------
+// spectral d/dx at (p,q): index swap along x, src/magnetic.f90:141-150
+template <typename TA>
+__device__ __forceinline__ double sdx(const TA* a, long base, int p, int q, int nx, const double* akx1)
+{
+    if (p == 0 || p == nx / 2) return 0.0;
+    const double v = (double)a[base + (long)q * nx + (nx - p)] * akx1[p];
+    return (p < nx / 2) ? -v : v;
+}
+// spectral d/dy at (p,q): index swap along y, src/magnetic.f90:155-164
+template <typename TA>
+__device__ __forceinline__ double sdy(const TA* a, long base, int p, int q, int nx, int ny, const double* aky1)
+{
+    if (q == 0 || q == ny / 2) return 0.0;
+    const double v = (double)a[base + (long)(ny - q) * nx + p] * aky1[q];
+    return (q < ny / 2) ? -v : v;
+}
+// 2/3 truncation band, src/magnetic.f90:666-717
+__device__ __forceinline__ bool in_band(int p, int q, int nx, int ny, int icx, int icy)
+{
+    return (p >= nx / 2 - icx && p <= nx / 2 + icx) || (q >= ny / 2 - icy && q <= ny / 2 + icy);
+}
+
+template <typename TW>
+__global__ void __launch_bounds__(NT) mag_spec1_kernel(MagArgs a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= a.g.n3) return;
+    const int nx = a.g.nx, ny = a.g.ny;
+    const int k = (int)(q / a.g.nxy);
+    const long base = (long)k * a.g.nxy;
+    const int rem = (int)(q - base);
+    const int jq = rem / nx, ip = rem - jq * nx;
+    const TW* ex = (const TW*)a.exh;
+    const TW* ey = (const TW*)a.eyh;
+    // 362-407: wk1 = dEx/dx + dEy/dy, wk2 = dEx/dy - dEy/dx
+    const double dxex = sdx(ex, base, ip, jq, nx, a.akx1), dxey = sdx(ey, base, ip, jq, nx, a.akx1);
+    const double dyex = sdy(ex, base, ip, jq, nx, ny, a.aky1), dyey = sdy(ey, base, ip, jq, nx, ny, a.aky1);
+    ((TW*)a.cc)[q] = (TW)(dyex - dxey);
+    ((TW*)a.d)[q] = (TW)(dxex + dyey);
+}
+
+// mean-field equations, src/magnetic.f90:328-352 (one workgroup)
+template <typename TW>
+__global__ void __launch_bounds__(1024) mean_emf_kernel(MagArgs a)
+{
+    extern __shared__ double sh[];
+    const int nz = a.g.nz;
+    double* w1d = sh;
+    double* w2d = sh + nz;
+    double* bxb = sh + 2 * nz;
+    double* byb = sh + 3 * nz;
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+        const long b = (long)k * a.g.nxy;
+        w1d[k] = a.p.xn * (double)((const TW*)a.exh)[b];
+        w2d[k] = a.p.xn * (double)((const TW*)a.eyh)[b];
+        bxb[k] = a.bxbar[k];
+        byb[k] = a.bybar[k];
+    }
+    __syncthreads();
+    const double h = a.p.ho12, s = a.p.sc1;
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+        const double dbx = dz1<double>(k, nz, 5, h, [&](int kk) { return w2d[kk]; });
+        const double dby = dz1<double>(k, nz, 5, h, [&](int kk) { return w1d[kk]; });
+        const double d2x = dz2<double>(k, nz, 3, s, [&](int kk) { return bxb[kk]; });
+        const double d2y = dz2<double>(k, nz, 3, s, [&](int kk) { return byb[kk]; });
+        a.bxbardot[k] = a.p.zck * d2x - dbx;
+        a.bybardot[k] = a.p.zck * d2y + dby;
+    }
+}
+
+// poldot, tordot, src/magnetic.f90:429-479
+template <typename TW>
+__global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= a.g.n3) return;
+    const int nx = a.g.nx, nxy = a.g.nxy, nz = a.g.nz;
+    const int k = (int)(q / nxy);
+    const long col = q - (long)k * nxy;
+    const int jq = (int)(col / nx), ip = (int)(col - (long)jq * nx);
+    const double k2 = a.akx2[ip] + a.aky2[jq];
+    const bool dc = (ip == 0 && jq == 0);
+    const double il_c = dc ? 0.0 : (double)((const TW*)a.cc)[q] / k2;
+    const double il_d = dc ? 0.0 : (double)((const TW*)a.dz)[q] / k2;
+    const double* pol = a.pol;
+    const double* tor = a.tor;
+    const double pzz = dz2<double>(k, nz, 3, a.p.sc1, [&](int kk) { return pol[col + (long)kk * nxy]; });
+    const double tzz = dz2<double>(k, nz, 3, a.p.sc1, [&](int kk) { return tor[col + (long)kk * nxy]; });
+    const double zck = a.p.zck;
+    a.poldot[q] = zck * pol[q] * k2 + zck * pzz + il_c;
+    a.tordot[q] = zck * tor[q] * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q];
+}
+
+// spectral B and J pieces of bandj, src/magnetic.f90:519-617
+template <typename TW>
+__global__ void __launch_bounds__(NT) bandj_spec_kernel(MagArgs a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= a.g.n3) return;
+    const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
+    const int k = (int)(q / nxy);
+    const long base = (long)k * nxy;
+    const int rem = (int)(q - base);
+    const int jq = rem / nx, ip = rem - jq * nx;
+    const TW* pz = (const TW*)a.polz;
+    const double* pol = a.pol;
+    const double* tor = a.tor;
+    const double bxh = sdx(pz, base, ip, jq, nx, a.akx1) + sdy(tor, base, ip, jq, nx, ny, a.aky1);
+    const double byh = sdy(pz, base, ip, jq, nx, ny, a.aky1) - sdx(tor, base, ip, jq, nx, a.akx1);
+    const double k2 = a.akx2[ip] + a.aky2[jq];
+    const double bzh = -1.0 * pol[q] * k2;
+    const double jzh = -1.0 * tor[q] * k2;
+    // d/dy and d/dx of bz-hat = -pol*(akx2+aky2) at the partner positions
+    double dyb = 0.0, dxb = 0.0;
+    if (!(jq == 0 || jq == ny / 2)) {
+        const int qq = ny - jq;
+        const double v = (-1.0 * pol[base + (long)qq * nx + ip] * (a.akx2[ip] + a.aky2[qq])) * a.aky1[jq];
+        dyb = (jq < ny / 2) ? -v : v;
+    }
+    if (!(ip == 0 || ip == nx / 2)) {
+        const int pp = nx - ip;
+        const double v = (-1.0 * pol[base + (long)jq * nx + pp] * (a.akx2[pp] + a.aky2[jq])) * a.akx1[ip];
+        dxb = (ip < nx / 2) ? -v : v;
+    }
+    ((TW*)a.bxh)[q] = (TW)bxh;
+    ((TW*)a.byh)[q] = (TW)byh;
+    ((TW*)a.bzh)[q] = (TW)bzh;
+    ((TW*)a.jzh)[q] = (TW)jzh;
+    ((TW*)a.dybzh)[q] = (TW)dyb;
+    ((TW*)a.dxbzh)[q] = (TW)dxb;
+}
+
+// J^2, src/magnetic.f90:630-645 (dybzh/dxbzh/jzh hold physical-space fields here)
+template <typename TW>
+__global__ void __launch_bounds__(NT) jsq_kernel(MagArgs a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= a.g.n3) return;
+    const int nxy = a.g.nxy, nz = a.g.nz;
+    const int k = (int)(q / nxy);
+    const long col = q - (long)k * nxy;
+    const TW* by = (const TW*)a.by;
+    const TW* bx = (const TW*)a.bx;
+    const double wk1 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)by[col + (long)kk * nxy]; });
+    const double wk2 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)bx[col + (long)kk * nxy]; });
+    const double ex = (double)((const TW*)a.dybzh)[q];
+    const double ey = (double)((const TW*)a.dxbzh)[q];
+    const double ez = (double)((const TW*)a.jzh)[q];
+    ((TW*)a.jsq)[q] = (TW)(ez * ez + ((ex - wk1) * (ex - wk1)) + ((wk2 - ey) * (wk2 - ey)));
+}
+
+// mag_init spectral part, src/magnetic.f90:63-233 (truncation folded into reads)
+template <typename TW>
+__global__ void __launch_bounds__(NT) mag_init_spec_kernel(MagArgs a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= a.g.n3) return;
+    const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
+    const int k = (int)(q / nxy);
+    const long base = (long)k * nxy;
+    const int rem = (int)(q - base);
+    const int jq = rem / nx, ip = rem - jq * nx;
+    const int icx = a.icutx, icy = a.icuty;
+    const TW* ex = (const TW*)a.exh;
+    const TW* ey = (const TW*)a.eyh;
+    const TW* ez = (const TW*)a.ezh;
+    auto T = [&](const TW* f, int p2, int q2) -> double {
+        return in_band(p2, q2, nx, ny, icx, icy) ? 0.0 : (double)f[base + (long)q2 * nx + p2];
+    };
+    const double k2 = a.akx2[ip] + a.aky2[jq];
+    const bool dc = (ip == 0 && jq == 0);
+    // pol = -Bz-hat / (akx2+aky2), 123-136
+    const double pol = dc ? 0.0 : -1.0 * T(ez, ip, jq) / k2;
+    // wk3 = dBy/dx - dBx/dy (spectral), 141-175
+    double dxby = 0.0, dybx = 0.0;
+    if (!(ip == 0 || ip == nx / 2)) {
+        const double v = T(ey, nx - ip, jq) * a.akx1[ip];
+        dxby = (ip < nx / 2) ? -v : v;
+    }
+    if (!(jq == 0 || jq == ny / 2)) {
+        const double v = T(ex, ip, ny - jq) * a.aky1[jq];
+        dybx = (jq < ny / 2) ? -v : v;
+    }
+    const double jz = dxby - dybx;
+    const double tor = dc ? 0.0 : -1.0 * jz / k2;
+    a.pol[q] = pol;
+    a.tor[q] = tor;
+    // 199-233: wk3 = -tor*(akx2+aky2), dBz/dy, dBz/dx from the truncated Bz-hat
+    ((TW*)a.jzh)[q] = (TW)(-1.0 * tor * k2);
+    double dyb = 0.0, dxb = 0.0;
+    if (!(jq == 0 || jq == ny / 2)) {
+        const double v = T(ez, ip, ny - jq) * a.aky1[jq];
+        dyb = (jq < ny / 2) ? -v : v;
+    }
+    if (!(ip == 0 || ip == nx / 2)) {
+        const double v = T(ez, nx - ip, jq) * a.akx1[ip];
+        dxb = (ip < nx / 2) ? -v : v;
+    }
+    ((TW*)a.dybzh)[q] = (TW)dyb;
+    ((TW*)a.dxbzh)[q] = (TW)dxb;
+    // 111-117: mean fields from the DC of each plane
+    if (ip == 0 && jq == 0) {
+        a.bxbar[k] = (double)ex[base] * a.p.xn;
+        a.bybar[k] = (double)ey[base] * a.p.xn;
+    }
+}
+
+__global__ void bzbar_kernel(MagArgs a, int wf32, double* out)
+{
+    const double v = wf32 ? (double)((const float*)a.ezh)[0] : ((const double*)a.ezh)[0];
+    out[0] = v * a.p.xn;
+}
+
+#define MHD_LAUNCH_W(kern, grid, block, shm, s, arg)                                       \
+    do {                                                                                   \
+        if ((arg).wf32) hipLaunchKernelGGL(kern<float>, grid, block, shm, s, arg);         \
+        else hipLaunchKernelGGL(kern<double>, grid, block, shm, s, arg);                   \
+        if (hipGetLastError() != hipSuccess) return -22;                                   \
+    } while (0)
+
+int launch_mag_spec1(const MagArgs& a, hipStream_t s)
+{
+    MHD_LAUNCH_W(mag_spec1_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    return 0;
+}
+int launch_mean_emf(const MagArgs& a, hipStream_t s)
+{
+    MHD_LAUNCH_W(mean_emf_kernel, dim3(1), dim3(256), 4 * a.g.nz * sizeof(double), s, a);
+    return 0;
+}
+int launch_mag_rhs(const MagArgs& a, hipStream_t s)
+{
+    MHD_LAUNCH_W(mag_rhs_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    return 0;
+}
+int launch_bandj_spec(const MagArgs& a, hipStream_t s)
+{
+    MHD_LAUNCH_W(bandj_spec_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    return 0;
+}
+int launch_jsq(const MagArgs& a, hipStream_t s)
+{
+    MHD_LAUNCH_W(jsq_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    return 0;
+}
+int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out)
+{
+    MHD_LAUNCH_W(mag_init_spec_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL(bzbar_kernel, dim3(1), dim3(1), 0, s, a, a.wf32, means_out);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+// ---------------------------------------------------------------------------
+// AB3 update, src/main.F90:413-477, plus magdealias (src/magnetic.f90:666-717)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) ab3_kernel(AB3Args a)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q < a.g.n3) {
+        const double c0 = a.c0, c1 = a.c1, c2 = a.c2;
+#pragma unroll
+        for (int f = 0; f < 7; ++f) {
+            if (f >= a.nfields) break;
+            double* x = a.var[f];
+            x[q] = x[q] + c0 * a.hist[f][2][q] + c1 * a.hist[f][1][q] + c2 * a.hist[f][0][q];
+        }
+        double* r = a.var[3];
+        if (r[q] < 0.05) {
+            r[q] = 0.05;
+            a.hist[3][0][q] = 0.0;
+            a.hist[3][1][q] = 0.0;
+            a.hist[3][2][q] = 0.0;
+        }
+        if (a.mag) {
+            const int nx = a.g.nx, ny = a.g.ny;
+            const int rem = (int)(q % a.g.nxy);
+            const int jq = rem / nx, ip = rem - jq * nx;
+            if (in_band(ip, jq, nx, ny, a.icutx, a.icuty)) {
+                a.var[5][q] = 0.0;
+                a.var[6][q] = 0.0;
+            }
+        }
+        if (a.shadow[0]) {
+            const int k = (int)(q / a.g.nxy);
+            ((float*)a.shadow[0])[q] = (float)a.var[0][q];
+            ((float*)a.shadow[1])[q] = (float)a.var[1][q];
+            ((float*)a.shadow[2])[q] = (float)a.var[2][q];
+            ((float*)a.shadow[3])[q] = (float)(a.var[3][q] - (a.bg_r ? a.bg_r[k] : 0.0));
+            ((float*)a.shadow[4])[q] = (float)(a.var[4][q] - (a.bg_t ? a.bg_t[k] : 0.0));
+        }
+    }
+    if (q < a.g.nz && a.mag) {
+        for (int f = 0; f < 2; ++f) {
+            double* x = a.mvar[f];
+            x[q] = x[q] + a.c0 * a.mhist[f][2][q] + a.c1 * a.mhist[f][1][q] + a.c2 * a.mhist[f][0][q];
+        }
+    }
+}
+
+int launch_ab3(const AB3Args& a, hipStream_t s)
+{
+    long n = a.g.n3 > a.g.nz ? a.g.n3 : a.g.nz;
+    hipLaunchKernelGGL(ab3_kernel, dim3(nblk(n)), dim3(NT), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+// ---------------------------------------------------------------------------
+// boundary conditions (defaults: impermeable stress-free, fixed T, vertical B)
+// src/dbydz.F90:3814-3900 (u, v, w), 4119 and 4183 (T), 4660-4750 (pol, tor,
+// means).  elf = 1/11.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) boundary_kernel(BCArgs a)
+{
+    const long p = (long)blockIdx.x * NT + threadIdx.x;
+    const long m = a.g.nxy;
+    const double elf = 1.0 / 11.0;
+    if (p < m && a.hydro) {
+        const long L = (long)(a.g.nz - 1) * m;
+        a.w[p] = 0.0;
+        a.u[p] = elf * (18.0 * a.u[m + p] - 9.0 * a.u[2 * m + p] + 2.0 * a.u[3 * m + p]);
+        a.v[p] = elf * (18.0 * a.v[m + p] - 9.0 * a.v[2 * m + p] + 2.0 * a.v[3 * m + p]);
+        a.t[p] = 1.0;
+        a.w[L + p] = 0.0;
+        a.u[L + p] = elf * (18.0 * a.u[L - m + p] - 9.0 * a.u[L - 2 * m + p] + 2.0 * a.u[L - 3 * m + p]);
+        a.v[L + p] = elf * (18.0 * a.v[L - m + p] - 9.0 * a.v[L - 2 * m + p] + 2.0 * a.v[L - 3 * m + p]);
+        a.t[L + p] = 1.0 + a.theta;
+    }
+    if (p < m && a.mag) {
+        const long L = (long)(a.g.nz - 1) * m;
+        {
+            a.tor[p] = 0.0;
+            a.pol[p] = elf * (18.0 * a.pol[m + p] - 9.0 * a.pol[2 * m + p] + 2.0 * a.pol[3 * m + p]);
+            a.tor[L + p] = 0.0;
+            a.pol[L + p] =
+                elf * (18.0 * a.pol[L - m + p] - 9.0 * a.pol[L - 2 * m + p] + 2.0 * a.pol[L - 3 * m + p]);
+        }
+    }
+    if (p == 0 && a.mag) {
+        a.bxbar[0] = 0.0;
+        a.bybar[0] = 0.0;
+        a.bxbar[a.g.nz - 1] = 0.0;
+        a.bybar[a.g.nz - 1] = 0.0;
+    }
+}
+
+int launch_boundary(const BCArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(boundary_kernel, dim3(nblk(a.g.nxy)), dim3(NT), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+__global__ void __launch_bounds__(NT) magdealias_kernel(Geom g, double* pol, double* tor, int icx, int icy)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= g.n3) return;
+    const int rem = (int)(q % g.nxy);
+    const int jq = rem / g.nx, ip = rem - jq * g.nx;
+    if (in_band(ip, jq, g.nx, g.ny, icx, icy)) {
+        pol[q] = 0.0;
+        tor[q] = 0.0;
+    }
+}
+
+int launch_magdealias(Geom g, double* pol, double* tor, int icutx, int icuty, hipStream_t s)
+{
+    hipLaunchKernelGGL(magdealias_kernel, dim3(nblk(g.n3)), dim3(NT), 0, s, g, pol, tor, icutx, icuty);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+// ---------------------------------------------------------------------------
+// reductions: one workgroup per plane writes partial[k][0..7]; one finishing
+// workgroup combines the planes in k order like the reference's loops
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&kind)[NV], double* sh)
+{
+    // kind: 0 sum, 1 min, 2 max
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sh[i * NT + tid] = v[i];
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if (tid < st) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const double a = sh[i * NT + tid], b = sh[i * NT + tid + st];
+                sh[i * NT + tid] = kind[i] == 0 ? a + b : kind[i] == 1 ? fmin(a, b) : fmax(a, b);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = sh[i * NT];
+}
+
+template <typename TW>
+__device__ __forceinline__ double rdw(const void* p, long q) { return (double)((const TW*)p)[q]; }
+
+template <typename TW>
+__global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
+{
+    __shared__ double sh[5 * NT];
+    const int k = blockIdx.x;
+    const long base = (long)k * a.g.nxy;
+    const int m = a.g.nxy;
+    const int tid = threadIdx.x;
+    if (a.op == RED_GLOBALAVG) {
+        // src/utils.F90:55-82: rav, urav, vrav, keav, bsqav
+        double v[5] = {0, 0, 0, 0, 0};
+        for (int i = tid; i < m; i += NT) {
+            const long q = base + i;
+            const double uu = a.u[q], vv = a.v[q], rr = a.r[q], ww = a.w[q];
+            const double bx = rdw<TW>(a.bx, q), by = rdw<TW>(a.by, q), bz = rdw<TW>(a.bz, q);
+            v[0] += rr;
+            v[1] += rr * uu;
+            v[2] += rr * vv;
+            v[3] += 0.5 * rr * (uu * uu + vv * vv + ww * ww);
+            v[4] += bx * bx + by * by + bz * bz;
+        }
+        const int kind[5] = {0, 0, 0, 0, 0};
+        block_reduce<5>(v, kind, sh);
+        if (tid == 0)
+            for (int i = 0; i < 5; ++i) a.partial[k * 8 + i] = v[i];
+    } else if (a.op == RED_MINMAX || a.op == RED_DIVB) {
+        double v[2] = {1e300, -1e300};
+        for (int i = tid; i < m; i += NT) {
+            const long q = base + i;
+            double x;
+            if (a.op == RED_MINMAX) x = a.f_f32 ? (double)((const float*)a.f0)[q] : ((const double*)a.f0)[q];
+            else
+                x = (a.f_f32 ? (double)((const float*)a.f0)[q] : ((const double*)a.f0)[q]) +
+                    (a.f_f32 ? (double)((const float*)a.f1)[q] : ((const double*)a.f1)[q]) +
+                    (a.f_f32 ? (double)((const float*)a.f2)[q] : ((const double*)a.f2)[q]);
+            v[0] = fmin(v[0], x);
+            v[1] = fmax(v[1], x);
+        }
+        const int kind[2] = {1, 2};
+        block_reduce<2>(v, kind, sh);
+        if (tid == 0) {
+            a.partial[k * 8 + 0] = v[0];
+            a.partial[k * 8 + 1] = v[1];
+        }
+    } else if (a.op == RED_CALCDT) {
+        // src/utils.F90:168-186
+        double v[3] = {1e300, -1.0e-50, -1.0e-50};
+        const double f = a.p.f, g1 = a.p.g1;
+        for (int i = tid; i < m; i += NT) {
+            const long q = base + i;
+            const double u0 = a.u[q], v0 = a.v[q], w0 = a.w[q], r0 = a.r[q], t0 = a.t[q];
+            const double bz0 = rdw<TW>(a.bz, q), bx0 = rdw<TW>(a.bx, q), by0 = rdw<TW>(a.by, q);
+            const double bsq0 = bx0 * bx0 + by0 * by0 + bz0 * bz0;
+            const double xx = sqrt(u0 * u0 + v0 * v0) + g1 * sqrt(t0) + sqrt((bsq0 * f) / r0);
+            const double xz = sqrt(w0 * w0) + g1 * sqrt(t0) + sqrt((bsq0 * f) / r0);
+            v[0] = fmin(v[0], r0);
+            v[1] = fmax(v[1], xx);
+            v[2] = fmax(v[2], xz);
+        }
+        const int kind[3] = {1, 2, 2};
+        block_reduce<3>(v, kind, sh);
+        if (tid == 0)
+            for (int i = 0; i < 3; ++i) a.partial[k * 8 + i] = v[i];
+    } else if (a.op == RED_NUSSELT) {
+        // src/dbydz.F90:4878-4886, bottom plane only (grid of 1)
+        double v[1] = {0.0};
+        const long L = (long)(a.g.nz - 1) * m;
+        const double c = 1.0 / (6.0 * a.p.dz);
+        for (int i = tid; i < m; i += NT)
+            v[0] += c * (11.0 * a.t[L + i] - 18.0 * a.t[L - m + i] + 9.0 * a.t[L - 2 * m + i] - 2.0 * a.t[L - 3 * m + i]);
+        const int kind[1] = {0};
+        block_reduce<1>(v, kind, sh);
+        if (tid == 0) a.partial[0] = v[0];
+    }
+}
+
+__global__ void reduce_finish_kernel(RedArgs a)
+{
+    if (threadIdx.x != 0) return;
+    const int nz = a.g.nz;
+    const double* P = a.partial;
+    double* R = a.result;
+    if (a.op == RED_GLOBALAVG) {
+        // src/utils.F90:83-127
+        double urbar = 0, vrbar = 0, rbar = 0, me = 0, ke = 0;
+        for (int k = 0; k < nz; ++k) {
+            const double* v = P + k * 8;
+            if (k == 0 || k == nz - 1) {
+                urbar += 0.5 * v[1];
+                vrbar += 0.5 * v[2];
+                rbar += 0.5 * v[0];
+                me += 0.25 * v[4];
+                ke += 0.5 * v[3];
+            } else {
+                urbar += v[1];
+                vrbar += v[2];
+                rbar += v[0];
+                me += 0.5 * v[4];
+                ke += v[3];
+            }
+        }
+        R[0] = urbar / rbar;
+        R[1] = vrbar / rbar;
+        R[2] = ke * a.p.xzn;
+        R[3] = me * a.p.xzn;
+    } else if (a.op == RED_MINMAX) {
+        double mn = P[0], mx = P[1];
+        for (int k = 1; k < nz; ++k) {
+            mn = fmin(mn, P[k * 8]);
+            mx = fmax(mx, P[k * 8 + 1]);
+        }
+        R[0] = mn;
+        R[1] = mx;
+    } else if (a.op == RED_DIVB) {
+        // src/utils.F90:270-288: divbmax starts at -1e30, divbmin at 1.0
+        double mx = -1.0e30, mn = 1.0;
+        for (int k = 0; k < nz; ++k) {
+            mx = fmax(mx, P[k * 8 + 1]);
+            mn = fmin(mn, P[k * 8]);
+        }
+        R[0] = mx;
+        R[1] = mn;
+    } else if (a.op == RED_CALCDT) {
+        // src/utils.F90:205-242
+        double denmn = P[0], vx = P[1], vz = P[2];
+        for (int k = 1; k < nz; ++k) {
+            denmn = fmin(denmn, P[k * 8]);
+            vx = fmax(vx, P[k * 8 + 1]);
+            vz = fmax(vz, P[k * 8 + 2]);
+        }
+        const double dtcflx = 0.16 * a.p.dd / vx;
+        const double dtcflz = 0.3 * a.p.dz / vz;
+        const double dtviscx = a.p.dtvisc0x * denmn, dtviscz = a.p.dtvisc0z * denmn;
+        const double dtdiffx = a.p.dtdiff0x * denmn, dtdiffz = a.p.dtdiff0z * denmn;
+        const double mn = fmin(fmin(fmin(dtcflx, dtcflz), fmin(dtviscx, dtviscz)), fmin(dtdiffx, dtdiffz));
+        R[0] = a.p.sf * mn;
+        R[1] = denmn;
+        R[2] = vx;
+        R[3] = vz;
+    } else if (a.op == RED_NUSSELT) {
+        const double tzbar = P[0] * a.p.xn;
+        R[0] = (tzbar - a.p.nusstmp) / (a.p.theta - a.p.nusstmp);
+    }
+}
+
+int launch_reduce(const RedArgs& a, hipStream_t s)
+{
+    const unsigned g = (a.op == RED_NUSSELT) ? 1u : (unsigned)a.g.nz;
+    if (a.wf32) hipLaunchKernelGGL(reduce_plane_kernel<float>, dim3(g), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL(reduce_plane_kernel<double>, dim3(g), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(64), 0, s, a);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+__global__ void com_shift_kernel(Geom g, double* u, double* v, const double* res)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= g.n3) return;
+    u[q] = u[q] - res[0];
+    v[q] = v[q] - res[1];
+}
+
+int launch_com_shift(Geom g, double* u, double* v, const double* res, hipStream_t s)
+{
+    hipLaunchKernelGGL(com_shift_kernel, dim3(nblk(g.n3)), dim3(NT), 0, s, g, u, v, res);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void copy_cvt_kernel(const void* src, int sf, void* dst, int df, long n)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= n) return;
+    const double v = sf ? (double)((const float*)src)[q] : ((const double*)src)[q];
+    if (df) ((float*)dst)[q] = (float)v;
+    else ((double*)dst)[q] = v;
+}
+
+int launch_copy_cvt(const void* src, int src_f32, void* dst, int dst_f32, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_cvt_kernel, dim3(nblk(n)), dim3(NT), 0, s, src, src_f32, dst, dst_f32, n);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+__global__ void fill_kernel(void* dst, int f32, double v, long n)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= n) return;
+    if (f32) ((float*)dst)[q] = (float)v;
+    else ((double*)dst)[q] = v;
+}
+
+int launch_fill(void* dst, int f32, double v, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(nblk(n)), dim3(NT), 0, s, dst, f32, v, n);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+}  // namespace mhd

@@ -1,0 +1,37 @@
+// spectral.cpp -- dispatch of the horizontal transform kernels by length.
+#include "spectral.hpp"
+
+namespace mhd {
+
+int launch_spectral_g0(int n, const SpecLaunch& L, int f32, hipStream_t s);
+int launch_spectral_g1(int n, const SpecLaunch& L, int f32, hipStream_t s);
+int launch_spectral_g2(int n, const SpecLaunch& L, int f32, hipStream_t s);
+int launch_spectral_g3(int n, const SpecLaunch& L, int f32, hipStream_t s);
+int launch_spectral_g4(int n, const SpecLaunch& L, int f32, hipStream_t s);
+int launch_spectral_g5(int n, const SpecLaunch& L, int f32, hipStream_t s);
+
+#define MHD_ALL_SIZES(X) X(8) X(12) X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(60) X(64) X(80) X(96) X(120) X(128) X(160) X(192) X(240) X(256) X(320) X(384) X(480) X(512) X(640) X(768) X(1024)
+
+bool spectral_len_supported(int n)
+{
+    switch (n) {
+#define X(n_) case n_: return true;
+        MHD_ALL_SIZES(X)
+#undef X
+    }
+    return false;
+}
+
+int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
+{
+    if (L.njobs <= 0) return 0;
+    const int n = (L.dir == 0) ? L.g.nx : L.g.ny;
+    if (n <= 60) return launch_spectral_g0(n, L, compute_f32, s);
+    if (n <= 128) return launch_spectral_g1(n, L, compute_f32, s);
+    if (n <= 256) return launch_spectral_g2(n, L, compute_f32, s);
+    if (n <= 480) return launch_spectral_g3(n, L, compute_f32, s);
+    if (n <= 640) return launch_spectral_g4(n, L, compute_f32, s);
+    return launch_spectral_g5(n, L, compute_f32, s);
+}
+
+}  // namespace mhd

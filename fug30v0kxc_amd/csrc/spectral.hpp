@@ -1,0 +1,49 @@
+// spectral.hpp -- host-side interface of the horizontal (x/y) transform
+// kernels: spectral derivatives (src/fft.F90 d1bydx/d1bydy/d2bydx/d2bydy),
+// 2/3 dealiasing (dealias), and the real<->halfcomplex 2-D transform passes
+// (forwardfft/inversefft) in FFTW-2 halfcomplex order.
+#pragma once
+
+#include "common.hpp"
+
+namespace mhd {
+
+enum SpecMode { SM_DERIV1 = 0, SM_DERIV2 = 1, SM_R2HC = 2, SM_HC2R = 3 };
+enum MulKind { MUL_D1 = 0, MUL_D2 = 1, MUL_MASK = 2 };
+enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2 };
+
+constexpr int SPEC_MAX_JOBS = 8;
+
+// One transform job: one input field (or a fused product of fields) and up
+// to two outputs.  Element types: *_f32 = 1 for float arrays, 0 for double.
+struct SpecJob {
+    const void* in[4];
+    int in_f32;
+    int load_op;      // LoadOp: in0 | scale*in0*in1 | in0*in1 - in2*in3
+    double scale;
+    const double* bg; // optional per-plane value subtracted on load (nz entries)
+    void* out[2];
+    int out_f32;
+    int mul[2];       // MulKind for out[0], out[1] (SM_DERIV*)
+    const double* addk; // optional per-plane value added on store (nz entries)
+    double adds;        // scalar added on store
+};
+
+struct SpecLaunch {
+    Geom g;
+    int dir;          // 0: along x (rows), 1: along y (columns)
+    int mode;         // SpecMode
+    int njobs;
+    SpecJob job[SPEC_MAX_JOBS];
+    const double* ak1;  // wavenumber tables of this direction (akx1/aky1)
+    const double* ak2;  // (akx2/aky2)
+    const void* tw;     // W_N^t table in the compute type
+    int icut;           // dealias cut-off (n/2)/3
+};
+
+// Returns 0 or a negative code (-20: unsupported length).
+int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s);
+
+bool spectral_len_supported(int n);
+
+}  // namespace mhd

@@ -1,0 +1,167 @@
+/*
+ * mhd.h -- C ABI of the MI355X (gfx950) per-timestep compressible-MHD core.
+ *
+ * Drop-in for the reference solver's time-step path (p3d "VarElecCond",
+ * Fortran 90).  The reference has no FFI: its step is a fixed sequence of
+ * argument-less module subroutines acting on module globals, called from
+ * `program pd3` (src/main.F90:340-747).  Each entry point below replaces one
+ * of those subroutines (cited per function); the host keeps the reference's
+ * loop structure, INPUT parameters and dump layout, and calls these through
+ * iso_c_binding (Fortran) or ctypes (Python).
+ *
+ * Conventions
+ *  - Host arrays are the reference's Fortran (nx,ny,nz) column-major FP64
+ *    buffers: element (i,j,k), 0-based, at i + nx*(j + ny*k).  They are
+ *    borrowed for the duration of the call only; device state is owned by
+ *    the context.
+ *  - Every function returns 0 on success.  Non-zero codes mirror the
+ *    reference's iret / abort_comms codes (MHD_E_*); mhd_strerror() names them.
+ *  - One context = one box on one GPU = one HIP stream.  Calls on one context
+ *    are not thread-safe; distinct contexts are independent.
+ */
+#ifndef MHD_H
+#define MHD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes (src/fft.F90:112-119, src/main.F90:216-219, src/dbydz.F90
+ * abort_comms(7), src/main.F90:589 abort_comms(12)) */
+enum {
+    MHD_OK = 0,
+    MHD_E_ODD_NX = -1,       /* init_fft iret = -1 */
+    MHD_E_ODD_NY = -2,       /* init_fft iret = -2 */
+    MHD_E_ALLOC = -5,        /* allocation failure */
+    MHD_E_NZ = 1,            /* fewer than six layers (abort_comms(1)) */
+    MHD_E_FLAG = 7,          /* unknown derivative flag (abort_comms(7)) */
+    MHD_E_DIVB = 12,         /* |div B| > 0.01 (abort_comms(12)) */
+    MHD_E_ARG = -10,         /* null or inconsistent argument */
+    MHD_E_SIZE = -20,        /* horizontal size has no transform kernel */
+    MHD_E_DEVICE = -30,      /* HIP runtime error */
+    MHD_E_STATE = -40        /* call out of order (e.g. step before state) */
+};
+
+enum { MHD_PREC_FP64 = 0, MHD_PREC_MIXED = 1 };
+
+/* Physical parameters after the INPUT flag-2 conversion (Ra, Q -> CK, F;
+ * src/initial.F90:183-195), in src/params.f90 order, plus extension knobs. */
+typedef struct mhd_params {
+    double gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax, sf;
+    double t_amp;   /* T noise amplitude of static() (reference hard-codes 0.1) */
+    double b_amp;   /* Bz noise amplitude (extension; 0 = reference) */
+} mhd_params_t;
+
+/* Driver scalars of src/main.F90 (dt history, AB start-up flags, counters). */
+typedef struct mhd_scalars {
+    double dt, dt1, dt2, ttime, bzbar;
+    int nit, iter, first, second;
+} mhd_scalars_t;
+
+/* Diagnostics of src/main.F90:552-650. */
+typedef struct mhd_diag {
+    double ubar, vbar;        /* GlobalAverages (already subtracted) */
+    double kinenergy;         /* kinenergy */
+    double magenergy;         /* F * menergy */
+    double nusselt;           /* nusselt(Nuss) */
+    double divbmax, divbmin;  /* checkdivb */
+    double dt;                /* current dt */
+    int nit;
+} mhd_diag_t;
+
+typedef struct mhd_ctx mhd_ctx_t;
+
+/* Ra, Q -> CK, F (INPUT flag 2), src/initial.F90:183-195 */
+void mhd_ra_q_to_ck_f(double gamma, double sigma, double theta, double cm, double zeta,
+                      double ra, double q, double* ck, double* f);
+
+/* set_constants + init_fft + init_dbydz + alloc_arrays
+ * (src/constants.f90:56-109, src/fft.F90:88-223, src/dbydz.F90:59-364,
+ * src/arrays.f90:81-300).  precision: MHD_PREC_FP64 or MHD_PREC_MIXED. */
+int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd_ctx_t** out);
+/* close_arrays + close_fft (src/arrays.f90:306-361, src/fft.F90:228-250) */
+void mhd_destroy(mhd_ctx_t* ctx);
+
+/* static + ini_trans_params + mag_init (src/initial.F90:25-86,
+ * src/trans_params.f90:15-69, src/magnetic.f90:5-279) */
+int mhd_static(mhd_ctx_t* ctx);
+/* ReadNetCDF's state hand-over + ini_trans_params + mag_init: load the 8
+ * physical fields (src/netcdf.F90 ReadNetCDF) and rebuild pol/tor on device */
+int mhd_set_state(mhd_ctx_t* ctx, const double* r, const double* t, const double* u,
+                  const double* v, const double* w, const double* bx, const double* by,
+                  const double* bz);
+/* the 8 dumped fields (WriteNetCDF, src/netcdf.F90:372-580); any may be NULL */
+int mhd_get_state(mhd_ctx_t* ctx, double* r, double* t, double* u, double* v, double* w,
+                  double* bx, double* by, double* bz);
+
+/* field access by id (MHD_F_*); level 1..3 selects an RHS history level
+ * (udot(:,:,:,level)); ignored for other fields.  Mean fields and their
+ * histories have nz entries, kappa/dkappa nz entries. */
+enum {
+    MHD_F_U = 0, MHD_F_V, MHD_F_W, MHD_F_T, MHD_F_R, MHD_F_BX, MHD_F_BY, MHD_F_BZ,
+    MHD_F_POL, MHD_F_TOR, MHD_F_JSQ,
+    MHD_F_UDOT, MHD_F_VDOT, MHD_F_WDOT, MHD_F_RDOT, MHD_F_TDOT, MHD_F_POLDOT, MHD_F_TORDOT,
+    MHD_F_BXBAR, MHD_F_BYBAR, MHD_F_BXBARDOT, MHD_F_BYBARDOT, MHD_F_KAPPA, MHD_F_DKAPPA,
+    MHD_F_COUNT
+};
+int mhd_get_field(mhd_ctx_t* ctx, int id, int level, double* dst);
+int mhd_set_field(mhd_ctx_t* ctx, int id, int level, const double* src);
+int mhd_get_scalars(mhd_ctx_t* ctx, mhd_scalars_t* s);
+int mhd_set_scalars(mhd_ctx_t* ctx, const mhd_scalars_t* s);
+
+/* The per-step subroutines of the reference, on device state.  The RHS
+ * routines write history level 3 (udot(:,:,:,3) etc.). */
+int mhd_hydro_deriv(mhd_ctx_t* ctx);   /* src/hydro_deriv.f90:5-505 */
+int mhd_mag_deriv(mhd_ctx_t* ctx);     /* src/magnetic.f90:285-487 */
+int mhd_bandj(mhd_ctx_t* ctx);         /* src/magnetic.f90:492-664 */
+int mhd_magdealias(mhd_ctx_t* ctx);    /* src/magnetic.f90:666-717 */
+int mhd_boundary(mhd_ctx_t* ctx);      /* src/dbydz.F90:3686-4197 (defaults) */
+int mhd_mag_boundary(mhd_ctx_t* ctx);  /* src/dbydz.F90:4205-4791 (vertical field) */
+int mhd_mag_init(mhd_ctx_t* ctx);      /* src/magnetic.f90:5-279 */
+
+/* nsteps iterations of the time-loop body src/main.F90:340-683:
+ * hydro_deriv, mag_deriv, AB3 coefficients and update (with Euler/AB2
+ * start-up, density floor), dealias every 10th step, magdealias, bandj,
+ * boundary, mag_boundary, diagnostics every ndu06 steps (ndu06 <= 0: never)
+ * and, when dt_fixed <= 0, calcdt every 10th step.  dt_fixed > 0 holds dt
+ * fixed.  `last` (may be NULL) receives the last diagnostics computed.
+ * Returns MHD_E_DIVB when checkdivb exceeds 0.01, as abort_comms(12). */
+int mhd_step(mhd_ctx_t* ctx, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* last);
+
+/* diagnostics (src/utils.F90, src/dbydz.F90:4796-4925) */
+int mhd_global_averages(mhd_ctx_t* ctx, double out[4]); /* ubar, vbar, ke, me (unscaled); subtracts ubar/vbar */
+int mhd_nusselt(mhd_ctx_t* ctx, double* nu);
+int mhd_checkdivb(mhd_ctx_t* ctx, double out[2]);       /* divbmax, divbmin */
+int mhd_sminmax(mhd_ctx_t* ctx, int field_id, double out[2]); /* min, max */
+int mhd_calcdt(mhd_ctx_t* ctx, double* dt);
+
+/* Operator hooks on host buffers (operators of src/fft.F90 and
+ * src/dbydz.F90).  in/out are (nx,ny,nz) arrays, or nz-vectors for the 1D
+ * ops; out2 is used by D2BYDX/D2BYDY (second derivative).  D1UPBYDZ reads
+ * the context's w as upwind selector, like the reference's global w. */
+enum {
+    MHD_OP_D1BYDX = 1, MHD_OP_D1BYDY, MHD_OP_D2BYDX, MHD_OP_D2BYDY,
+    MHD_OP_D1BYDZ, MHD_OP_D1UPBYDZ, MHD_OP_D2BYDZ, MHD_OP_DEALIAS,
+    MHD_OP_FORWARDFFT, MHD_OP_INVERSEFFT, MHD_OP_D1BYDZ1D, MHD_OP_D2BYDZ1D
+};
+int mhd_op(mhd_ctx_t* ctx, int op, const double* in, double* out, double* out2, int flag);
+
+/* Event timing of one kernel family over subsequent mhd_step calls (HIP
+ * events recorded on the context's stream around every launch of it).
+ * kernel: MHD_K_*; enable=1 resets and starts, enable=0 stops.  Query with
+ * mhd_kernel_time(): total milliseconds and number of launches. */
+enum { MHD_K_NONE = 0, MHD_K_XDERIV2 = 1, MHD_K_YDERIV2 = 2, MHD_K_ZDERIV = 3, MHD_K_HYDRO_A = 4,
+       MHD_K_AB3 = 5 };
+int mhd_kernel_timing(mhd_ctx_t* ctx, int kernel, int enable);
+int mhd_kernel_time(mhd_ctx_t* ctx, double* total_ms, long* launches);
+
+/* Waits for all work of the context; returns any pending device error. */
+int mhd_sync(mhd_ctx_t* ctx);
+/* Bytes of device memory held by the context. */
+long mhd_device_bytes(mhd_ctx_t* ctx);
+const char* mhd_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
