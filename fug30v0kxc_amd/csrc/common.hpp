@@ -56,6 +56,20 @@ __device__ __forceinline__ void stv(void* p, size_t i, int f32, TC v)
     else ((double*)p)[i] = (double)v;
 }
 
+// The by-value launch struct of a kernel, read in place from the kernarg
+// segment (constant address space).  Indexing its arrays with a
+// workgroup-uniform index then compiles to scalar loads instead of a private
+// copy of the whole struct.  T must be the kernel's first argument.
+template <typename T>
+__device__ __forceinline__ const T& kernarg()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const T*)(__builtin_amdgcn_kernarg_segment_ptr());
+#else
+    return *(const T*)nullptr;   // host pass: never executed
+#endif
+}
+
 // Box geometry passed by value to every kernel.
 struct Geom {
     int nx, ny, nz;

@@ -15,38 +15,132 @@ static inline unsigned nblk(long n, int nt = NT) { return (unsigned)((n + nt - 1
 // ---------------------------------------------------------------------------
 // batched z-derivatives
 // ---------------------------------------------------------------------------
-template <typename TC>
-__global__ void __launch_bounds__(NT) z_kernel(ZLaunch L)
+// A thread owns one (i,j) column and KC consecutive planes.  Chunks clear of
+// the boundary rows hold planes k0-3 .. k0+KC+2 in a register window and
+// apply the interior stencils at compile-time offsets; the chunks touching
+// the first or last three planes apply the row-specific stencils.
+constexpr int ZKC = 16;
+
+// TC: compute/output type; TIN: input element type; KIND: ZKind; PROD: the
+// input is scale*in*in1.  w (upwind selector) has the element type TW.
+template <typename TC, typename TIN, typename TW, int KIND, bool PROD>
+__global__ void __launch_bounds__(NT) z_kernel(ZLaunch Lv)
 {
-    const ZJob& J = L.job[blockIdx.y];
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
-    if (q >= L.g.n3) return;
+    (void)Lv;
+    const ZLaunch& L = kernarg<ZLaunch>();
+    const ZJob& J = L.job[blockIdx.z];
     const int nxy = L.g.nxy, nz = L.g.nz;
-    const int k = (int)(q / nxy);
-    const long col = q - (long)k * nxy;
-    auto F = [&](int kk) -> TC {
+    const long col = (long)blockIdx.x * NT + threadIdx.x;
+    if (col >= nxy) return;
+    const TIN* __restrict__ in = (const TIN*)J.in;
+    const TIN* __restrict__ in1 = (const TIN*)J.in1;
+    const TC scale = (TC)J.scale;
+    TC* __restrict__ out = (TC*)J.out;
+    const int flag = J.flag;
+    const TW* __restrict__ wp = (const TW*)L.w;
+    const TC h = (TC)L.ho12, s = (TC)L.sc1;
+    auto ld = [=](int kk) -> TC {
         const long i = col + (long)kk * nxy;
-        if (J.load_op == LD_PROD)
-            return ((TC)J.scale * ldv<TC>(J.in, i, J.in_f32)) * ldv<TC>(J.in1, i, J.in_f32);
-        return ldv<TC>(J.in, i, J.in_f32);
+        TC v = (TC)in[i];
+        if constexpr (PROD) v = (scale * v) * (TC)in1[i];
+        return v;
     };
-    TC d;
-    if (J.kind == ZK_D1) d = dz1<TC>(k, nz, J.flag, (TC)L.ho12, F);
-    else if (J.kind == ZK_D2) d = dz2<TC>(k, nz, J.flag, (TC)L.sc1, F);
-    else d = dz1up<TC>(k, nz, J.flag, (TC)L.ho12, ldv<TC>(L.w, q, L.w_f32), F);
-    stv<TC>(J.out, q, J.out_f32, d);
+    const int k0 = blockIdx.y * ZKC;
+    if (k0 >= 3 && k0 + ZKC <= nz - 3) {
+        constexpr int LO = (KIND == ZK_D1UP) ? 0 : 1;          // first window slot used
+        constexpr int HI = (KIND == ZK_D1UP) ? ZKC + 6 : ZKC + 5;
+        TC win[ZKC + 6];
+#pragma unroll
+        for (int t = LO; t < HI; ++t) win[t] = ld(k0 - 3 + t);
+#pragma unroll
+        for (int t = 0; t < ZKC; ++t) {
+            const int c = t + 3;
+            const long o = col + (long)(k0 + t) * nxy;
+            TC d;
+            if constexpr (KIND == ZK_D1) {
+                d = h * (win[c - 2] - (TC)8 * win[c - 1] + (TC)8 * win[c + 1] - win[c + 2]);
+            } else if constexpr (KIND == ZK_D2) {
+                d = s * (-win[c - 2] + (TC)16 * win[c - 1] - (TC)30 * win[c] + (TC)16 * win[c + 1] - win[c + 2]);
+            } else {
+                const TC wv = (TC)wp[o];
+                d = (wv >= (TC)0)
+                        ? h * (-win[c - 3] + (TC)6 * win[c - 2] - (TC)18 * win[c - 1] + (TC)10 * win[c] +
+                               (TC)3 * win[c + 1])
+                        : h * (win[c + 3] - (TC)6 * win[c + 2] + (TC)18 * win[c + 1] - (TC)10 * win[c] -
+                               (TC)3 * win[c - 1]);
+            }
+            out[o] = d;
+        }
+    } else {
+        const int k1 = (k0 + ZKC < nz) ? k0 + ZKC : nz;
+        for (int k = k0; k < k1; ++k) {
+            const long o = col + (long)k * nxy;
+            TC d;
+            if constexpr (KIND == ZK_D1) d = dz1<TC>(k, nz, flag, h, ld);
+            else if constexpr (KIND == ZK_D2) d = dz2<TC>(k, nz, flag, s, ld);
+            else d = dz1up<TC>(k, nz, flag, h, (TC)wp[o], ld);
+            out[o] = d;
+        }
+    }
 }
 
+template <typename TC, typename TIN, typename TW, int KIND, bool PROD>
+static int launch_z_group(ZLaunch& G, hipStream_t s)
+{
+    dim3 grid(nblk(G.g.nxy), (unsigned)((G.g.nz + ZKC - 1) / ZKC), G.njobs);
+    hipLaunchKernelGGL((z_kernel<TC, TIN, TW, KIND, PROD>), grid, dim3(NT), 0, s, G);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+template <typename TC, typename TIN, typename TW>
+static int launch_z_kind(ZLaunch& G, int kind, bool prod, hipStream_t s)
+{
+    switch (kind * 2 + (prod ? 1 : 0)) {
+    case 0: return launch_z_group<TC, TIN, TW, ZK_D1, false>(G, s);
+    case 1: return launch_z_group<TC, TIN, TW, ZK_D1, true>(G, s);
+    case 2: return launch_z_group<TC, TIN, TW, ZK_D1UP, false>(G, s);
+    case 3: return launch_z_group<TC, TIN, TW, ZK_D1UP, true>(G, s);
+    case 4: return launch_z_group<TC, TIN, TW, ZK_D2, false>(G, s);
+    case 5: return launch_z_group<TC, TIN, TW, ZK_D2, true>(G, s);
+    }
+    return -23;
+}
+
+// Jobs are grouped by (input type, kind, product load); one launch per group.
 int launch_z(const ZLaunch& L, hipStream_t s)
 {
     if (L.njobs <= 0) return 0;
     for (int i = 0; i < L.njobs; ++i)
         if (!zflag_ok(L.job[i].kind, L.job[i].flag)) return 7;
-    dim3 grid(nblk(L.g.n3), L.njobs);
-    bool f32 = L.job[0].out_f32 != 0;
-    if (f32) hipLaunchKernelGGL(z_kernel<float>, grid, dim3(NT), 0, s, L);
-    else hipLaunchKernelGGL(z_kernel<double>, grid, dim3(NT), 0, s, L);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
+    bool done[Z_MAX_JOBS] = {};
+    for (int i = 0; i < L.njobs; ++i) {
+        if (done[i]) continue;
+        const ZJob& a = L.job[i];
+        ZLaunch G = L;
+        G.njobs = 0;
+        for (int j = i; j < L.njobs; ++j) {
+            const ZJob& b = L.job[j];
+            if (!done[j] && b.in_f32 == a.in_f32 && b.kind == a.kind && b.load_op == a.load_op &&
+                b.out_f32 == a.out_f32) {
+                G.job[G.njobs++] = b;
+                done[j] = true;
+            }
+        }
+        const bool prod = a.load_op == LD_PROD;
+        int e;
+        if (a.out_f32) {
+            if (L.w_f32) e = a.in_f32 ? launch_z_kind<float, float, float>(G, a.kind, prod, s)
+                                      : launch_z_kind<float, double, float>(G, a.kind, prod, s);
+            else e = a.in_f32 ? launch_z_kind<float, float, double>(G, a.kind, prod, s)
+                              : launch_z_kind<float, double, double>(G, a.kind, prod, s);
+        } else {
+            if (a.in_f32) return -24;   // FP32 input with FP64 output is not used
+            e = L.w_f32 ? launch_z_kind<double, double, float>(G, a.kind, prod, s)
+                        : launch_z_kind<double, double, double>(G, a.kind, prod, s);
+        }
+        if (e) return e;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
