@@ -1,0 +1,242 @@
+// fft_reg.hpp -- register-blocked complex FFTs for gfx950.
+//
+// A transform of length N = R_0 R_1 ... R_{P-1} runs as P self-sorting
+// (Stockham) passes.  In every pass a thread owns one radix-R butterfly with
+// all R points in registers; between passes the points move through LDS
+// once (write, barrier, read, barrier).  Pass 0 takes its inputs
+// x[j + r*N/R_0] straight from memory and the last pass leaves
+// X[j + r*N/R_{P-1}] in registers, so a spectral multiply and the inverse
+// transform -- run with the radix plan reversed, whose first pass consumes
+// exactly those registers -- need no LDS round trip between them, and the
+// inverse's last pass again holds x[j + r*N/R_0], ready to store.
+//
+// LDS indices are padded by one complex per 256 bytes (pad()), which keeps
+// the stride-R writes of the passes on distinct banks.
+#pragma once
+
+#include "fft_lds.hpp"
+
+namespace mhd {
+
+// radix plans (forward order); every radix is one of 2, 3, 4, 5, 8, 16
+template <int N>
+struct FftPlan;
+#define MHD_PLAN(N_, ...)                                        \
+    template <>                                                  \
+    struct FftPlan<N_> {                                         \
+        static constexpr int N = N_;                             \
+        static constexpr int R[] = {__VA_ARGS__};                \
+        static constexpr int P = sizeof(R) / sizeof(int);        \
+    };
+// Radix-4-heavy plans: with the pipelined kernel, small butterflies (few
+// registers, high occupancy) beat radix-16 ones although they add LDS passes
+// (measured at 256x256x160 FP64: x d1+d2 243 us with 4x4x4x4 vs 265 us with
+// 16x16; x d1 143 us vs 162 us).
+MHD_PLAN(8, 8)
+MHD_PLAN(12, 4, 3)
+MHD_PLAN(16, 4, 4)
+MHD_PLAN(20, 4, 5)
+MHD_PLAN(24, 8, 3)
+MHD_PLAN(30, 2, 3, 5)
+MHD_PLAN(32, 4, 8)
+MHD_PLAN(40, 8, 5)
+MHD_PLAN(48, 4, 4, 3)
+MHD_PLAN(60, 4, 3, 5)
+MHD_PLAN(64, 4, 4, 4)
+MHD_PLAN(80, 4, 4, 5)
+MHD_PLAN(96, 4, 8, 3)
+MHD_PLAN(120, 8, 3, 5)
+MHD_PLAN(128, 4, 4, 8)
+MHD_PLAN(160, 8, 4, 5)
+MHD_PLAN(192, 4, 4, 4, 3)
+MHD_PLAN(240, 4, 4, 3, 5)
+MHD_PLAN(256, 4, 4, 4, 4)
+MHD_PLAN(320, 4, 4, 4, 5)
+MHD_PLAN(384, 4, 4, 8, 3)
+MHD_PLAN(480, 8, 4, 3, 5)
+MHD_PLAN(512, 4, 4, 4, 8)
+MHD_PLAN(640, 4, 4, 8, 5)
+MHD_PLAN(768, 4, 4, 4, 4, 3)
+MHD_PLAN(1024, 4, 4, 4, 4, 4)
+#undef MHD_PLAN
+
+// An explicit plan (the autotuner's alternatives use these).
+template <int N_, int... Rs>
+struct Plan {
+    static constexpr int N = N_;
+    static constexpr int R[] = {Rs...};
+    static constexpr int P = sizeof...(Rs);
+};
+
+template <typename PL>
+constexpr int plan_radix(int p, bool inv)
+{
+    return inv ? PL::R[PL::P - 1 - p] : PL::R[p];
+}
+// product of the radices before pass p
+template <typename PL>
+constexpr int plan_ns(int p, bool inv)
+{
+    int ns = 1;
+    for (int q = 0; q < p; ++q) ns *= plan_radix<PL>(q, inv);
+    return ns;
+}
+// threads per transform: the largest butterfly count of any pass
+template <typename PL>
+constexpr int plan_tpt()
+{
+    int t = 1;
+    for (int p = 0; p < PL::P; ++p)
+        if (PL::N / PL::R[p] > t) t = PL::N / PL::R[p];
+    return t;
+}
+template <typename PL>
+constexpr int plan_rmax()
+{
+    int r = 1;
+    for (int p = 0; p < PL::P; ++p)
+        if (PL::R[p] > r) r = PL::R[p];
+    return r;
+}
+
+// W_8^k and W_16^k twiddle multiplies (forward = exp(-i...))
+template <typename T, bool INV>
+__device__ __forceinline__ cpx<T> rot(cpx<T> a, T c, T s)
+{
+    // a * (c - i s) forward, a * (c + i s) inverse
+    if (INV) return {a.x * c - a.y * s, a.y * c + a.x * s};
+    return {a.x * c + a.y * s, a.y * c - a.x * s};
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(cpx<T>* v)
+{
+    cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    butterfly<T, 4, INV>(e);
+    butterfly<T, 4, INV>(o);
+    const T h = (T)0.70710678118654752440084436210485;
+    o[1] = rot<T, INV>(o[1], h, h);
+    o[2] = cmul_i<T, INV ? 1 : -1>(o[2]);
+    o[3] = rot<T, INV>(o[3], -h, h);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft16(cpx<T>* v)
+{
+    cpx<T> e[8], o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        e[k] = v[2 * k];
+        o[k] = v[2 * k + 1];
+    }
+    dft8<T, INV>(e);
+    dft8<T, INV>(o);
+    const T c1 = (T)0.92387953251128675612818318939679;  // cos(pi/8)
+    const T s1 = (T)0.38268343236508977172845998403040;  // sin(pi/8)
+    const T h = (T)0.70710678118654752440084436210485;
+    o[1] = rot<T, INV>(o[1], c1, s1);
+    o[2] = rot<T, INV>(o[2], h, h);
+    o[3] = rot<T, INV>(o[3], s1, c1);
+    o[4] = cmul_i<T, INV ? 1 : -1>(o[4]);
+    o[5] = rot<T, INV>(o[5], -s1, c1);
+    o[6] = rot<T, INV>(o[6], -h, h);
+    o[7] = rot<T, INV>(o[7], -c1, s1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 8] = csub(e[k], o[k]);
+    }
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft_r(cpx<T>* v)
+{
+    if constexpr (R == 8) dft8<T, INV>(v);
+    else if constexpr (R == 16) dft16<T, INV>(v);
+    else butterfly<T, R, INV>(v);
+}
+
+template <typename T, typename PL>
+struct RegFFT {
+    static constexpr int N = PL::N;
+    static constexpr int P = PL::P;
+    static constexpr int TPT = plan_tpt<PL>();
+    static constexpr int RMAX = plan_rmax<PL>();
+    static constexpr int R0 = PL::R[0];
+    static constexpr int RL = PL::R[P - 1];
+    static constexpr int NB0 = N / R0;   // butterflies of the first forward pass
+    static constexpr int NBL = N / RL;   // butterflies of the last forward pass
+    static constexpr int PADSH = sizeof(T) == 8 ? 4 : 5;   // one pad per 256 bytes
+    static constexpr int SP = N + (N >> PADSH) + 1;        // LDS stride per transform
+
+    static __device__ __forceinline__ int pad(int i) { return i + (i >> PADSH); }
+
+    // butterfly of pass p on v (thread's butterfly index j)
+    template <int p, bool INV>
+    static __device__ __forceinline__ void pass(cpx<T>* v, int j, const cpx<T>* __restrict__ tw)
+    {
+        constexpr int R = plan_radix<PL>(p, INV);
+        constexpr int Ns = plan_ns<PL>(p, INV);
+        if constexpr (Ns > 1) {
+            const int kk = j % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const cpx<T> w = tw[r * kk * (N / (Ns * R))];
+                v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+            }
+        }
+        dft_r<T, R, INV>(v);
+    }
+
+    // outputs of pass p -> LDS -> inputs of pass p+1
+    template <int p, bool INV>
+    static __device__ __forceinline__ void exchange(cpx<T>* v, int j, cpx<T>* lds)
+    {
+        constexpr int R = plan_radix<PL>(p, INV);
+        constexpr int Ns = plan_ns<PL>(p, INV);
+        constexpr int NB = N / R;
+        if (j < NB) {
+            const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[pad(base + r * Ns)] = v[r];
+        }
+        __syncthreads();
+        constexpr int R2 = plan_radix<PL>(p + 1, INV);
+        constexpr int NB2 = N / R2;
+        if (j < NB2) {
+#pragma unroll
+            for (int r = 0; r < R2; ++r) v[r] = lds[pad(j + r * NB2)];
+        }
+        __syncthreads();
+    }
+
+    template <int p, bool INV>
+    static __device__ __forceinline__ void run_from(cpx<T>* v, int j, cpx<T>* lds, const cpx<T>* __restrict__ tw)
+    {
+        if constexpr (p < P) {
+            constexpr int NB = N / plan_radix<PL>(p, INV);
+            if (j < NB) pass<p, INV>(v, j, tw);
+            if constexpr (p + 1 < P) exchange<p, INV>(v, j, lds);
+            run_from<p + 1, INV>(v, j, lds, tw);
+        }
+    }
+
+    // forward: v holds x[j + r*NB0] (j < NB0) -> X[j + r*NBL] (j < NBL)
+    static __device__ __forceinline__ void forward(cpx<T>* v, int j, cpx<T>* lds, const cpx<T>* __restrict__ tw)
+    {
+        run_from<0, false>(v, j, lds, tw);
+    }
+    // backward (unnormalised): v holds X[j + r*NBL] -> x[j + r*NB0]
+    static __device__ __forceinline__ void inverse(cpx<T>* v, int j, cpx<T>* lds, const cpx<T>* __restrict__ tw)
+    {
+        run_from<0, true>(v, j, lds, tw);
+    }
+};
+
+}  // namespace mhd

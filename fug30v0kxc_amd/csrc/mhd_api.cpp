@@ -69,6 +69,7 @@ struct mhd_ctx {
     void* wk[W_COUNT];
     double *red_partial, *red_result, *d_bzbar;
     double* stage;     // device FP64 staging (one field)
+    int max_groups;    // resident workgroups for the persistent transform kernels
     double* hbuf;      // pinned host staging (one field)
 
     mhd_scalars_t sc;
@@ -207,10 +208,10 @@ SpecLaunch spec_base(mhd_ctx* c, int dir, int mode)
     L.g = c->g;
     L.dir = dir;
     L.mode = mode;
-    L.ak1 = dir == 0 ? c->akx1 : c->aky1;
-    L.ak2 = dir == 0 ? c->akx2 : c->aky2;
+    L.cx = 8.0 * std::atan(1.0) / (dir == 0 ? c->p.xmax : c->p.ymax);   // src/fft.F90:184-185
     L.tw = dir == 0 ? c->twx : c->twy;
     L.icut = dir == 0 ? c->icutx : c->icuty;
+    L.max_groups = c->max_groups;
     return L;
 }
 
@@ -740,8 +741,11 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     if (nz < 6) return MHD_E_NZ;
     if (!spectral_len_supported(nx) || !spectral_len_supported(ny)) return MHD_E_SIZE;
     if (precision != MHD_PREC_FP64) return MHD_E_ARG;
-    int ndev = 0;
+    int ndev = 0, dev = 0, ncu = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MHD_E_DEVICE;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return MHD_E_DEVICE;
 
     mhd_ctx* c = new mhd_ctx();
     memset((void*)c, 0, sizeof(*c));
@@ -758,6 +762,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     set_constants(c);
     c->ring[0] = 0; c->ring[1] = 1; c->ring[2] = 2;
     c->sc.first = 1;
+    c->max_groups = 8 * ncu;   // enough resident workgroups to keep every CU busy
 
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return MHD_E_DEVICE; }
 

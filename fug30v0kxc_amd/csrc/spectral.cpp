@@ -22,9 +22,8 @@ bool spectral_len_supported(int n)
     return false;
 }
 
-int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
+static int launch_group(const SpecLaunch& L, int compute_f32, hipStream_t s)
 {
-    if (L.njobs <= 0) return 0;
     const int n = (L.dir == 0) ? L.g.nx : L.g.ny;
     if (n <= 60) return launch_spectral_g0(n, L, compute_f32, s);
     if (n <= 128) return launch_spectral_g1(n, L, compute_f32, s);
@@ -32,6 +31,27 @@ int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
     if (n <= 480) return launch_spectral_g3(n, L, compute_f32, s);
     if (n <= 640) return launch_spectral_g4(n, L, compute_f32, s);
     return launch_spectral_g5(n, L, compute_f32, s);
+}
+
+// The load operation is a compile-time parameter of the kernels, so jobs are
+// launched in groups of equal load operation.
+int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
+{
+    if (L.njobs <= 0) return 0;
+    bool done[SPEC_MAX_JOBS] = {};
+    for (int i = 0; i < L.njobs; ++i) {
+        if (done[i]) continue;
+        SpecLaunch G = L;
+        G.njobs = 0;
+        for (int j = i; j < L.njobs; ++j)
+            if (!done[j] && L.job[j].load_op == L.job[i].load_op) {
+                G.job[G.njobs++] = L.job[j];
+                done[j] = true;
+            }
+        const int e = launch_group(G, compute_f32, s);
+        if (e) return e;
+    }
+    return 0;
 }
 
 }  // namespace mhd

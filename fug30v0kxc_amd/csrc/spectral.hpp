@@ -10,7 +10,9 @@ namespace mhd {
 
 enum SpecMode { SM_DERIV1 = 0, SM_DERIV2 = 1, SM_R2HC = 2, SM_HC2R = 3 };
 enum MulKind { MUL_D1 = 0, MUL_D2 = 1, MUL_MASK = 2 };
-enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2 };
+// LD_BG: FP64 input minus a per-plane background (the mixed path's
+// fluctuation loads); the others read inputs of the compute type.
+enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3 };
 
 constexpr int SPEC_MAX_JOBS = 8;
 
@@ -35,10 +37,10 @@ struct SpecLaunch {
     int mode;         // SpecMode
     int njobs;
     SpecJob job[SPEC_MAX_JOBS];
-    const double* ak1;  // wavenumber tables of this direction (akx1/aky1)
-    const double* ak2;  // (akx2/aky2)
-    const void* tw;     // W_N^t table in the compute type
-    int icut;           // dealias cut-off (n/2)/3
+    double cx;        // 2*pi/xmax (dir x) or 2*pi/ymax (dir y): mode m has wavenumber m*cx
+    const void* tw;   // W_N^t table in the compute type
+    int icut;         // dealias cut-off (n/2)/3
+    int max_groups;   // resident workgroups to launch (persistent loop); 0 = one per item
 };
 
 // Returns 0 or a negative code (-20: unsupported length).

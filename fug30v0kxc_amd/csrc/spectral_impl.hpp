@@ -7,245 +7,341 @@
 // the 2/3 dealias, and the two halves of the 2-D real transforms
 // (forwardfft: r2hc per line; inversefft: hc2r per line).
 //
-// A workgroup owns 2*TPW real lines (rows for dir=x, columns for dir=y) of one
-// job and packs line pairs (a, b) into TPW complex lines a + i b held in LDS.
-// Because every operator here multiplies each mode by a Hermitian-symmetric
-// factor m(N-k) = conj(m(k)), one complex FFT, one multiply and one inverse
-// FFT return the operator applied to a in the real part and to b in the
+// A workgroup owns B transforms of one job; each transform packs two real
+// lines (rows for dir=x, adjacent columns for dir=y) as a + i b.  Because
+// every operator here multiplies each mode by a Hermitian-symmetric factor
+// m(N-k) = conj(m(k)), one complex FFT, one multiply and one inverse FFT
+// return the operator applied to a in the real part and to b in the
 // imaginary part; real<->halfcomplex conversions separate / merge the pair
-// explicitly.  Each line is read once from HBM and each output written once.
+// through one extra LDS round trip.  Each line is read once from HBM and each
+// output written once, straight from registers (see fft_reg.hpp).
 //
-// Rows (dir=x) are contiguous, so a workgroup reads 2*TPW*N consecutive
-// elements.  Columns (dir=y) are read as tiles of 2*TPW adjacent columns,
-// i.e. 2*TPW*sizeof(T)-byte segments per row.
-#include "fft_lds.hpp"
+// Rows (dir=x): the TPT threads of a transform read TPT consecutive elements
+// of each of its two rows per load.  Columns (dir=y): consecutive threads
+// belong to consecutive column pairs, so one load reads 2*B adjacent columns
+// of one row (2*B*sizeof(T) bytes).
+#include "fft_reg.hpp"
 #include "spectral.hpp"
 
 namespace mhd {
 
-template <typename TC>
-__device__ __forceinline__ TC load_job(const SpecJob& J, long i, int k)
-{
-    if (J.load_op == LD_PLAIN) {
-        if (J.bg) return (TC)(ldv<double>(J.in[0], i, J.in_f32) - J.bg[k]);
-        return ldv<TC>(J.in[0], i, J.in_f32);
-    } else if (J.load_op == LD_PROD) {
-        TC a = ldv<TC>(J.in[0], i, J.in_f32), b = ldv<TC>(J.in[1], i, J.in_f32);
-        return ((TC)J.scale * a) * b;
-    } else {
-        TC a = ldv<TC>(J.in[0], i, J.in_f32), b = ldv<TC>(J.in[1], i, J.in_f32);
-        TC c = ldv<TC>(J.in[2], i, J.in_f32), d = ldv<TC>(J.in[3], i, J.in_f32);
-        return a * b - c * d;
+// Input loaders, fixed at compile time per launch.  Inputs have the compute
+// type TC, except LD_BG, which reads the FP64 state and subtracts a per-plane
+// background in FP64 before rounding to TC.
+template <typename TC, int LOP>
+struct Loader {
+    const TC* __restrict__ a;
+    const TC* __restrict__ b;
+    const TC* __restrict__ c;
+    const TC* __restrict__ d;
+    const double* __restrict__ s64;
+    TC scale;
+    double bgk;
+    __device__ __forceinline__ Loader(const SpecJob& J, int k)
+    {
+        a = (const TC*)J.in[0];
+        b = (const TC*)J.in[1];
+        c = (const TC*)J.in[2];
+        d = (const TC*)J.in[3];
+        s64 = (const double*)J.in[0];
+        scale = (TC)J.scale;
+        bgk = (LOP == LD_BG && J.bg) ? J.bg[k] : 0.0;
     }
-}
+    __device__ __forceinline__ TC operator()(long i) const
+    {
+        if constexpr (LOP == LD_PLAIN) return a[i];
+        else if constexpr (LOP == LD_BG) return (TC)(s64[i] - bgk);
+        else if constexpr (LOP == LD_PROD) return (scale * a[i]) * b[i];
+        else return a[i] * b[i] - c[i] * d[i];
+    }
+};
 
-// spectral multiplier of mode k (full complex spectrum index), folded with 1/N
+// spectral multiplier of mode k (full complex spectrum index), folded with
+// 1/N.  The wavenumbers are those of src/fft.F90:196-221: mode m = min(k,N-k)
+// has akx1 = m*cx and akx2 = -(m*m*cx*cx), Nyquist included in akx2 only.
 template <typename TC, int N>
-__device__ __forceinline__ cpx<TC> apply_mul(int kind, cpx<TC> z, int k, const double* __restrict__ ak1,
-                                             const double* __restrict__ ak2, int icut)
+__device__ __forceinline__ cpx<TC> apply_mul(int kind, cpx<TC> z, int k, double cx, int icut)
 {
     constexpr double invN = 1.0 / N;
+    const int m = (k < N - k) ? k : N - k;
     if (kind == MUL_D1) {
         if (k == 0 || k == N / 2) return {(TC)0, (TC)0};
-        const TC a = (TC)(ak1[k] * invN);
+        const TC a = (TC)(((double)m * cx) * invN);
         // +i*a for k < N/2; the negative frequency N-k carries -i*a
         if (k < N / 2) return {-a * z.y, a * z.x};
         return {a * z.y, -a * z.x};
     } else if (kind == MUL_D2) {
-        const TC a = (TC)(ak2[k] * invN);
+        const TC a = (TC)((-((double)m * (double)m * cx * cx)) * invN);
         return {a * z.x, a * z.y};
     } else {
-        const int m = (k < N - k) ? k : N - k;
         if (m < N / 2 - icut) return {z.x * (TC)invN, z.y * (TC)invN};
         return {(TC)0, (TC)0};
     }
 }
 
-template <int DIR, int N, int TPW>
-struct LineMap {
-    // element e of a workgroup's block -> (line in WG, position n, global index, plane, valid)
-    static __device__ __forceinline__ void map(const Geom& g, int e, int& line, int& n, long& gi, int& k,
-                                               bool& ok)
+template <typename PL, int DIR, int B>
+struct Lines {
+    static constexpr int N = PL::N;
+    static constexpr int TPT = plan_tpt<PL>();
+    // (transform t, butterfly j) of this thread
+    static __device__ __forceinline__ void lane(int tid, int& t, int& j)
     {
-        constexpr int LINES = 2 * TPW;
+        if constexpr (DIR == 0) { j = tid % TPT; t = tid / TPT; }
+        else { t = tid % B; j = tid / B; }
+    }
+    // line groups per job: B complex lines each
+    static __host__ __device__ __forceinline__ long groups(const Geom& g)
+    {
+        if constexpr (DIR == 0) return ((long)g.nz * g.ny / 2 + B - 1) / B;
+        else return (long)g.nz * (g.nx / (2 * B));
+    }
+    // global index of element 0 of the transform's lines a and b in group grp
+    static __device__ __forceinline__ bool base(const Geom& g, long grp, int t, long& ia, long& ib, long& step,
+                                                int& k)
+    {
         if constexpr (DIR == 0) {
-            line = e / N;
-            n = e - line * N;
-            const long gl = (long)blockIdx.x * LINES + line;
-            ok = gl < (long)g.nz * g.ny;
-            gi = gl * N + n;
-            k = (int)(gl / g.ny);
+            const long cl = grp * B + t;
+            const long row = 2 * cl;
+            k = (int)(row / g.ny);
+            ia = row * N;
+            ib = ia + N;
+            step = 1;
+            return cl < (long)g.nz * g.ny / 2;
         } else {
-            n = e / LINES;
-            line = e - n * LINES;
-            const int tpp = g.nx / LINES;
-            k = blockIdx.x / tpp;
-            const int x0 = (blockIdx.x - k * tpp) * LINES;
-            gi = (long)k * g.nxy + (long)n * g.nx + x0 + line;
-            ok = true;
+            const int tpp = g.nx / (2 * B);
+            k = (int)(grp / tpp);
+            const int x0 = (int)(grp - (long)k * tpp) * 2 * B;
+            ia = (long)k * g.nxy + x0 + 2 * t;
+            ib = ia + 1;
+            step = g.nx;
+            return true;
         }
     }
 };
 
-template <typename TC, int N, int DIR, int TPW, int MODE, int NT>
-__global__ void __launch_bounds__(NT) spec_kernel(SpecLaunch L)
+// Persistent, software-pipelined kernel: workgroup w handles the items
+// (job, line group) w, w + gridDim.x, ...  The pass-0 inputs of the next item
+// are loaded into registers before the current item is transformed, and the
+// transform touches only registers and LDS (twiddles are staged in LDS), so
+// the loads of item i+1 are in flight while item i is computed and stored.
+template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
+__global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
 {
-    constexpr int NOUT = (MODE == SM_DERIV2) ? 2 : 1;
-    constexpr int S = (DIR == 0) ? N : N + 1;
-    constexpr int LINES = 2 * TPW;
-    constexpr int NE = LINES * N;
-    constexpr int NBIN = TPW * N;
-    extern __shared__ __align__(16) unsigned char smem[];
-    cpx<TC>* buf = reinterpret_cast<cpx<TC>*>(smem);
-    TC* rbuf = reinterpret_cast<TC*>(smem);
-    const SpecJob& J = L.job[blockIdx.y];
+    (void)Lv;
+    constexpr int N = PL::N;
+    using F = RegFFT<TC, PL>;
+    using LN = Lines<PL, DIR, B>;
+    constexpr int NT = B * plan_tpt<PL>();
+    constexpr int NB0 = F::NB0, NBL = F::NBL, R0 = F::R0, RL = F::RL;
+    const SpecLaunch& L = kernarg<SpecLaunch>();
     const Geom g = L.g;
-    const int tid = threadIdx.x;
-    const cpx<TC>* __restrict__ tw = reinterpret_cast<const cpx<TC>*>(L.tw);
-
-    // ---- load 2*TPW real lines into TPW complex lines --------------------
-#pragma unroll 4
-    for (int e = tid; e < NE; e += NT) {
-        int line, n, k;
-        long gi;
-        bool ok;
-        LineMap<DIR, N, TPW>::map(g, e, line, n, gi, k, ok);
-        TC v = (TC)0;
-        if (ok) v = (MODE == SM_HC2R) ? ldv<TC>(J.in[0], gi, J.in_f32) : load_job<TC>(J, gi, k);
-        rbuf[((line >> 1) * S + n) * 2 + (line & 1)] = v;
-    }
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<TC>* twl = reinterpret_cast<cpx<TC>*>(smem);
+    const cpx<TC>* __restrict__ twg = reinterpret_cast<const cpx<TC>*>(L.tw);
+    for (int i = threadIdx.x; i < N; i += NT) twl[i] = twg[i];
+    int t, j;
+    LN::lane(threadIdx.x, t, j);
+    cpx<TC>* lds = twl + N + t * F::SP;
+    const long ngrp = LN::groups(g);
+    const long nitems = ngrp * L.njobs;
+    const double cx = L.cx;
+    const int icut = L.icut;
     __syncthreads();
 
-    if constexpr (MODE == SM_HC2R) {
-        // merge two halfcomplex lines (a in .x, b in .y) into Z = A + iB
-        constexpr int PER = (NBIN + NT - 1) / NT;
-        cpx<TC> z[PER];
+    // pass-0 inputs of one item (zeros for lines past the end)
+    auto fetch = [&](long item, cpx<TC>* dst) {
+        const int jb = (int)(item / ngrp);
+        const long grp = item - (long)jb * ngrp;
+        const SpecJob& J = L.job[jb];
+        long ia, ib, st;
+        int k;
+        const bool ok = LN::base(g, grp, t, ia, ib, st, k);
+        if constexpr (MODE == SM_HC2R) {
+            const TC* __restrict__ src = (const TC*)J.in[0];
 #pragma unroll
-        for (int p = 0; p < PER; ++p) {
-            const int b = tid + p * NT;
-            if (NBIN % NT == 0 || b < NBIN) {
-                const int t = b / N, kk = b - (b / N) * N;
-                const cpx<TC> u = buf[t * S + kk];
-                const cpx<TC> m = buf[t * S + ((N - kk) % N)];
-                cpx<TC> A, B;
-                if (kk == 0 || kk == N / 2) { A = {u.x, (TC)0}; B = {u.y, (TC)0}; }
-                else if (kk < N / 2) { A = {u.x, m.x}; B = {u.y, m.y}; }
-                else { A = {m.x, -u.x}; B = {m.y, -u.y}; }
-                z[p] = {A.x - B.y, A.y + B.x};
+            for (int r = 0; r < R0; ++r) {
+                const int n = j + r * NB0;
+                dst[r] = {(TC)0, (TC)0};
+                if (ok && j < NB0) dst[r] = {src[ia + n * st], src[ib + n * st]};
+            }
+        } else {
+            const Loader<TC, LOP> ld(J, ok ? k : 0);
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+                const int n = j + r * NB0;
+                dst[r] = {(TC)0, (TC)0};
+                if (ok && j < NB0) dst[r] = {ld(ia + n * st), ld(ib + n * st)};
             }
         }
-        __syncthreads();
+    };
+
+    cpx<TC> nxt[R0];
+    long item = blockIdx.x;
+    if (item < nitems) fetch(item, nxt);
+    for (; item < nitems; item += gridDim.x) {
+        cpx<TC> v[F::RMAX];
 #pragma unroll
-        for (int p = 0; p < PER; ++p) {
-            const int b = tid + p * NT;
-            if (NBIN % NT == 0 || b < NBIN) buf[(b / N) * S + (b - (b / N) * N)] = z[p];
-        }
-        __syncthreads();
-        LdsFFT<TC, N, TPW, NT, S, true>::run(buf, tw, tid);
-    } else {
-        LdsFFT<TC, N, TPW, NT, S, false>::run(buf, tw, tid);
-    }
+        for (int r = 0; r < R0; ++r) v[r] = nxt[r];
+        if (item + gridDim.x < nitems) fetch(item + gridDim.x, nxt);
 
-    if constexpr (MODE == SM_DERIV1 || MODE == SM_DERIV2) {
-        for (int b = tid; b < NBIN; b += NT) {
-            const int t = b / N, kk = b - (b / N) * N;
-            const cpx<TC> z = buf[t * S + kk];
-            if constexpr (NOUT == 2) buf[TPW * S + t * S + kk] = apply_mul<TC, N>(J.mul[1], z, kk, L.ak1, L.ak2, L.icut);
-            buf[t * S + kk] = apply_mul<TC, N>(J.mul[0], z, kk, L.ak1, L.ak2, L.icut);
-        }
-        __syncthreads();
-        LdsFFT<TC, N, NOUT * TPW, NT, S, true>::run(buf, tw, tid);
-    }
+        const int jb = (int)(item / ngrp);
+        const long grp = item - (long)jb * ngrp;
+        const SpecJob& J = L.job[jb];
+        long ia, ib, st;
+        int k;
+        const bool ok = LN::base(g, grp, t, ia, ib, st, k);
+        TC* __restrict__ out0 = (TC*)J.out[0];
+        TC* __restrict__ out1 = (TC*)J.out[1];
+        // epilogue constant of this line pair (one plane): the mean parts of B
+        const TC add = (J.addk && ok ? (TC)J.addk[k] : (TC)0) + (TC)J.adds;
 
-    // ---- store -----------------------------------------------------------
-#pragma unroll 2
-    for (int e = tid; e < NE; e += NT) {
-        int line, n, k;
-        long gi;
-        bool ok;
-        LineMap<DIR, N, TPW>::map(g, e, line, n, gi, k, ok);
-        if (!ok) continue;
-        if constexpr (MODE == SM_R2HC) {
-            const int t = line >> 1;
-            const int kk = (n <= N / 2) ? n : N - n;
-            const cpx<TC> zk = buf[t * S + kk];
-            const cpx<TC> zn = buf[t * S + ((N - kk) % N)];
-            TC v;
-            if ((line & 1) == 0) v = (n <= N / 2) ? (TC)0.5 * (zk.x + zn.x) : (TC)0.5 * (zk.y - zn.y);
-            else v = (n <= N / 2) ? (TC)0.5 * (zk.y + zn.y) : (TC)0.5 * (zn.x - zk.x);
-            stv<TC>(J.out[0], gi, J.out_f32, v);
+        if constexpr (MODE == SM_HC2R) {
+            // raw halfcomplex lines -> LDS (natural order), then Z = A + iB
+            if (j < NB0) {
+#pragma unroll
+                for (int r = 0; r < R0; ++r) lds[F::pad(j + r * NB0)] = v[r];
+            }
+            __syncthreads();
+            if (j < NBL) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    const int kk = j + r * NBL;
+                    const cpx<TC> u = lds[F::pad(kk)];
+                    const cpx<TC> m = lds[F::pad((N - kk) % N)];
+                    cpx<TC> A, Bv;
+                    if (kk == 0 || kk == N / 2) { A = {u.x, (TC)0}; Bv = {u.y, (TC)0}; }
+                    else if (kk < N / 2) { A = {u.x, m.x}; Bv = {u.y, m.y}; }
+                    else { A = {m.x, -u.x}; Bv = {m.y, -u.y}; }
+                    v[r] = {A.x - Bv.y, A.y + Bv.x};
+                }
+            }
+            __syncthreads();
+            F::inverse(v, j, lds, twl);
+            if (ok && j < NB0) {
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const int n = j + r * NB0;
+                    out0[ia + n * st] = v[r].x * (TC)(1.0 / N) + add;
+                    out0[ib + n * st] = v[r].y * (TC)(1.0 / N) + add;
+                }
+            }
         } else {
+            F::forward(v, j, lds, twl);
+            if constexpr (MODE == SM_R2HC) {
+                // spectrum -> LDS, then separate the pair into two halfcomplex lines
+                if (j < NBL) {
 #pragma unroll
-            for (int o = 0; o < NOUT; ++o) {
-                TC v = rbuf[((o * TPW + (line >> 1)) * S + n) * 2 + (line & 1)];
-                if constexpr (MODE == SM_HC2R) v = v * (TC)(1.0 / N);
-                if (J.addk) v = v + (TC)J.addk[k];
-                if (J.adds != 0.0) v = v + (TC)J.adds;
-                stv<TC>(J.out[o], gi, J.out_f32, v);
+                    for (int r = 0; r < RL; ++r) lds[F::pad(j + r * NBL)] = v[r];
+                }
+                __syncthreads();
+                if (ok && j < NB0) {
+#pragma unroll
+                    for (int r = 0; r < R0; ++r) {
+                        const int n = j + r * NB0;
+                        const int kk = (n <= N / 2) ? n : N - n;
+                        const cpx<TC> zk = lds[F::pad(kk)];
+                        const cpx<TC> zn = lds[F::pad((N - kk) % N)];
+                        TC a, b;
+                        if (n <= N / 2) { a = (TC)0.5 * (zk.x + zn.x); b = (TC)0.5 * (zk.y + zn.y); }
+                        else { a = (TC)0.5 * (zk.y - zn.y); b = (TC)0.5 * (zn.x - zk.x); }
+                        out0[ia + n * st] = a;
+                        out0[ib + n * st] = b;
+                    }
+                }
+                __syncthreads();
+            } else {
+                constexpr int NOUT = (MODE == SM_DERIV2) ? 2 : 1;
+                cpx<TC> z[RL];
+                if constexpr (NOUT == 2) {
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) z[r] = v[r];
+                }
+#pragma unroll
+                for (int o = 0; o < NOUT; ++o) {
+                    if (j < NBL) {
+#pragma unroll
+                        for (int r = 0; r < RL; ++r)
+                            v[r] = apply_mul<TC, N>(J.mul[o], o == 0 ? v[r] : z[r], j + r * NBL, cx, icut);
+                    }
+                    F::inverse(v, j, lds, twl);
+                    if (ok && j < NB0) {
+                        TC* __restrict__ out = (o == 0) ? out0 : out1;
+#pragma unroll
+                        for (int r = 0; r < R0; ++r) {
+                            const int n = j + r * NB0;
+                            out[ia + n * st] = v[r].x + add;
+                            out[ib + n * st] = v[r].y + add;
+                        }
+                    }
+                }
             }
         }
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // host dispatch
 // ---------------------------------------------------------------------------
-template <int N>
-constexpr int x_tpw() { return (1024 / N) < 2 ? 2 : (1024 / N) > 16 ? 16 : (1024 / N); }
+// transforms per workgroup: about 256 threads along x; along y, 8 (FP64) or
+// 16 (FP32) column pairs = 128-byte row segments; LDS per workgroup <= 64 KB.
+template <typename TC, typename PL, int DIR>
+constexpr int spec_b()
+{
+    constexpr int tpt = plan_tpt<PL>();
+    constexpr int sp = RegFFT<TC, PL>::SP * (int)sizeof(cpx<TC>);
+    constexpr int twb = PL::N * (int)sizeof(cpx<TC>);
+    int b = DIR == 0 ? (128 / tpt > 0 ? 128 / tpt : 1) : (sizeof(TC) == 8 ? 8 : 16);
+    while (b > 1 && (b * tpt > 1024 || twb + b * sp > 65536)) b /= 2;
+    return b;
+}
 
-template <typename TC, int N, int DIR, int TPW, int MODE>
+template <typename TC, typename PL, int B>
+constexpr size_t spec_lds_bytes()
+{
+    return (size_t)(PL::N + B * RegFFT<TC, PL>::SP) * sizeof(cpx<TC>);
+}
+
+template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
 static int launch_one(const SpecLaunch& L, hipStream_t s)
 {
-    constexpr int NT = 256;
-    constexpr int NOUT = (MODE == SM_DERIV2) ? 2 : 1;
-    constexpr int S = (DIR == 0) ? N : N + 1;
-    const size_t lds = (size_t)NOUT * TPW * S * sizeof(cpx<TC>);
-    auto kern = spec_kernel<TC, N, DIR, TPW, MODE, NT>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return -21;
-    }
-    dim3 grid;
-    if (DIR == 0) grid.x = (unsigned)(((long)L.g.nz * L.g.ny + 2 * TPW - 1) / (2 * TPW));
-    else grid.x = (unsigned)(L.g.nz * (L.g.nx / (2 * TPW)));
-    grid.y = L.njobs;
-    hipLaunchKernelGGL(kern, grid, dim3(NT), lds, s, L);
+    constexpr int NT = B * plan_tpt<PL>();
+    constexpr size_t lds = spec_lds_bytes<TC, PL, B>();
+    static_assert(lds <= 65536, "transform workgroup exceeds 64 KB of LDS");
+    auto kern = spec_kernel<TC, PL, DIR, B, MODE, LOP>;
+    const long items = Lines<PL, DIR, B>::groups(L.g) * L.njobs;
+    long grid = items;
+    if (L.max_groups > 0 && grid > L.max_groups) grid = L.max_groups;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, s, L);
     return hipGetLastError() == hipSuccess ? 0 : -22;
 }
 
-template <typename TC, int N, int DIR, int TPW>
+// (mode, load op) pairs the solver uses; every job of L has the same load op
+template <typename TC, typename PL, int DIR, int B>
 static int launch_mode(const SpecLaunch& L, hipStream_t s)
 {
-    switch (L.mode) {
-    case SM_DERIV1: return launch_one<TC, N, DIR, TPW, SM_DERIV1>(L, s);
-    case SM_DERIV2: return launch_one<TC, N, DIR, TPW, SM_DERIV2>(L, s);
-    case SM_R2HC: return launch_one<TC, N, DIR, TPW, SM_R2HC>(L, s);
-    case SM_HC2R: return launch_one<TC, N, DIR, TPW, SM_HC2R>(L, s);
+    const int lop = L.job[0].load_op;
+    switch (L.mode * 8 + lop) {
+    case SM_DERIV1 * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_PLAIN>(L, s);
+    case SM_DERIV1 * 8 + LD_PROD: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_PROD>(L, s);
+    case SM_DERIV1 * 8 + LD_BG: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_BG>(L, s);
+    case SM_DERIV2 * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_DERIV2, LD_PLAIN>(L, s);
+    case SM_DERIV2 * 8 + LD_BG: return launch_one<TC, PL, DIR, B, SM_DERIV2, LD_BG>(L, s);
+    case SM_R2HC * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_R2HC, LD_PLAIN>(L, s);
+    case SM_R2HC * 8 + LD_CROSS: return launch_one<TC, PL, DIR, B, SM_R2HC, LD_CROSS>(L, s);
+    case SM_HC2R * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_HC2R, LD_PLAIN>(L, s);
     }
     return -23;
-}
-
-// Column tiles: the widest tile (2*TPW columns) whose LDS footprint fits
-// 160 KB and whose width divides nx.  FP64 tiles are 16 columns (128 B per
-// row), FP32 tiles 32 columns.
-template <typename TC, int N>
-constexpr int y_tpw_max()
-{
-    constexpr int cap = sizeof(TC) == 8 ? 8 : 16;
-    constexpr int fit = (160 * 1024) / (2 * (N + 1) * (int)sizeof(cpx<TC>));
-    return fit >= cap ? cap : fit >= 8 ? 8 : fit >= 4 ? 4 : fit >= 2 ? 2 : 1;
 }
 
 template <typename TC, int N>
 static int launch_n(const SpecLaunch& L, hipStream_t s)
 {
-    if (L.dir == 0) return launch_mode<TC, N, 0, x_tpw<N>()>(L, s);
-    constexpr int T0 = y_tpw_max<TC, N>();
-    if (L.g.nx % (2 * T0) == 0) return launch_mode<TC, N, 1, T0>(L, s);
-    if constexpr (T0 > 4) {
-        if (L.g.nx % 8 == 0) return launch_mode<TC, N, 1, 4>(L, s);
-    }
-    return launch_mode<TC, N, 1, 1>(L, s);
+    using PL = FftPlan<N>;
+    if (L.dir == 0) return launch_mode<TC, PL, 0, spec_b<TC, PL, 0>()>(L, s);
+    constexpr int B0 = spec_b<TC, PL, 1>();
+    if (L.g.nx % (2 * B0) == 0) return launch_mode<TC, PL, 1, B0>(L, s);
+    return launch_mode<TC, PL, 1, 1>(L, s);
 }
 
 }  // namespace mhd
