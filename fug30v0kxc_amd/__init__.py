@@ -22,7 +22,7 @@ from . import _abi
 from ._abi import (FIELD_IDS, HISTORY_FIELDS, KERNELS, MHD_PREC_FP64, MHD_PREC_MIXED, OPS,
                    VECTOR_FIELDS, MhdDiag, MhdParams, MhdScalars)
 
-__all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "MHD_PREC_FP64",
+__all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "write_dump", "read_dump", "MHD_PREC_FP64",
            "MHD_PREC_MIXED"]
 
 
@@ -63,6 +63,45 @@ def reference_params(ra=1e5, q=100.0, t_amp=1e-3, b_amp=0.0, gamma=5.0 / 3.0, si
     ck, f = ra_q_to_ck_f(gamma, sigma, theta, cm, zeta, ra, q)
     return dict(gamma=gamma, sigma=sigma, theta=theta, ck=ck, cm=cm, f=f, tayl=tayl, psi=psi,
                 zeta=zeta, xmax=xmax, ymax=ymax, sf=sf, t_amp=t_amp, b_amp=b_amp)
+
+
+DUMP_FIELDS = ("r", "t", "u", "v", "w", "bx", "by", "bz")   # R T U V W Bx By Bz
+
+
+def write_dump(path, fields, iteration=0, time=0.0, timestep=0.0, params=None, iprec=8,
+               history="", version=3):
+    """Write a T16-style dump (WriteNetCDF layout, netCDF classic).  `fields`
+    maps r, t, u, v, w, bx, by, bz to (nz, ny, nx) arrays."""
+    arrs = [_f64(fields[n]) for n in DUMP_FIELDS]
+    nz, ny, nx = arrs[0].shape
+    m = _abi.MhdDump()
+    m.nx, m.ny, m.nz, m.version, m.iteration = nx, ny, nz, version, iteration
+    m.time, m.timestep = time, timestep
+    for k, v in (params or {}).items():
+        if hasattr(m, k) and k not in ("nx", "ny", "nz"):
+            setattr(m, k, v)
+    m.history = history.encode()[:255]
+    ptrs = (ctypes.POINTER(ctypes.c_double) * 8)(*[_dp(a) for a in arrs])
+    code = _abi.load().mhd_write_dump(str(path).encode(), ctypes.byref(m), ptrs, iprec)
+    if code:
+        raise MhdError(code, f"write_dump({path})")
+
+
+def read_dump(path):
+    """(fields, meta) of a dump/restart file; fields as (nz, ny, nx) float64."""
+    L = _abi.load()
+    m = _abi.MhdDump()
+    code = L.mhd_read_dump(str(path).encode(), ctypes.byref(m), None)
+    if code:
+        raise MhdError(code, f"read_dump({path})")
+    arrs = [np.empty((m.nz, m.ny, m.nx)) for _ in DUMP_FIELDS]
+    ptrs = (ctypes.POINTER(ctypes.c_double) * 8)(*[_dp(a) for a in arrs])
+    code = L.mhd_read_dump(str(path).encode(), ctypes.byref(m), ptrs)
+    if code:
+        raise MhdError(code, f"read_dump({path})")
+    meta = {n: getattr(m, n) for n, _ in _abi.MhdDump._fields_ if n != "history"}
+    meta["history"] = m.history.decode(errors="replace")
+    return dict(zip(DUMP_FIELDS, arrs)), meta
 
 
 class Solver:

@@ -60,6 +60,15 @@ class MhdDiag(ctypes.Structure):
                 ("divbmin", ctypes.c_double), ("dt", ctypes.c_double), ("nit", ctypes.c_int)]
 
 
+class MhdDump(ctypes.Structure):
+    _fields_ = ([("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                 ("version", ctypes.c_int), ("iteration", ctypes.c_int),
+                 ("time", ctypes.c_double), ("timestep", ctypes.c_double)]
+                + [(n, ctypes.c_double) for n in ("gamma", "sigma", "theta", "ck", "cm", "f", "tayl",
+                                                   "psi", "zeta", "xmax", "ymax")]
+                + [("history", ctypes.c_char * 256)])
+
+
 _lib = None
 
 
@@ -104,6 +113,8 @@ def load():
         "mhd_kernel_time": (i, [vp, dp, ctypes.POINTER(ctypes.c_long)]),
         "mhd_sync": (i, [vp]),
         "mhd_device_bytes": (ctypes.c_long, [vp]),
+        "mhd_write_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp), i]),
+        "mhd_read_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -1,0 +1,335 @@
+// netcdf_io.cpp -- dump / restart files in the reference's layout, written
+// and read without libnetcdf.
+//
+// Format: netCDF classic (CDF-1, 32-bit offsets, big-endian), as created by
+// the reference's CreateNetCDF / WriteNetCDF (src/netcdf.F90:39-580) with
+// itype = 1: dimensions nx, ny, nz, one; global attributes version (int),
+// history (text), gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax
+// (double); variables time(one) double with units="UNKNOWN",
+// iteration(one) int, timestep(one) double, then R, T, U, V, W, Bx, By, Bz
+// over (nz, ny, nx) as double (iprec 8) or float (iprec 4).  The reader
+// accepts any classic file holding those variables (double or float) and
+// any attribute/dimension order.
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd.h"
+
+namespace {
+
+enum { NC_BYTE = 1, NC_CHAR = 2, NC_SHORT = 3, NC_INT = 4, NC_FLOAT = 5, NC_DOUBLE = 6 };
+enum { NC_DIMENSION = 0x0A, NC_VARIABLE = 0x0B, NC_ATTRIBUTE = 0x0C };
+
+const char* kFields[8] = {"R", "T", "U", "V", "W", "Bx", "By", "Bz"};
+const char* kAttrs[11] = {"gamma", "sigma", "theta", "ck", "cm", "f", "tayl", "psi", "zeta", "xmax", "ymax"};
+
+struct Out {
+    std::vector<unsigned char> b;
+    void u32(uint32_t v)
+    {
+        for (int s = 24; s >= 0; s -= 8) b.push_back((unsigned char)(v >> s));
+    }
+    void i32(int32_t v) { u32((uint32_t)v); }
+    void f64(double d)
+    {
+        uint64_t v;
+        memcpy(&v, &d, 8);
+        for (int s = 56; s >= 0; s -= 8) b.push_back((unsigned char)(v >> s));
+    }
+    void f32(float f)
+    {
+        uint32_t v;
+        memcpy(&v, &f, 4);
+        u32(v);
+    }
+    void pad4()
+    {
+        while (b.size() % 4) b.push_back(0);
+    }
+    void name(const std::string& s)
+    {
+        u32((uint32_t)s.size());
+        b.insert(b.end(), s.begin(), s.end());
+        pad4();
+    }
+};
+
+size_t type_size(int t)
+{
+    switch (t) {
+    case NC_BYTE:
+    case NC_CHAR: return 1;
+    case NC_SHORT: return 2;
+    case NC_INT:
+    case NC_FLOAT: return 4;
+    case NC_DOUBLE: return 8;
+    }
+    return 0;
+}
+
+struct In {
+    const unsigned char* p;
+    size_t n, o = 0;
+    bool ok = true;
+    uint32_t u32()
+    {
+        if (o + 4 > n) { ok = false; return 0; }
+        uint32_t v = ((uint32_t)p[o] << 24) | ((uint32_t)p[o + 1] << 16) | ((uint32_t)p[o + 2] << 8) | p[o + 3];
+        o += 4;
+        return v;
+    }
+    std::string name()
+    {
+        uint32_t len = u32();
+        if (!ok || o + len > n) { ok = false; return ""; }
+        std::string s((const char*)p + o, len);
+        o += (len + 3) & ~3u;
+        return s;
+    }
+    void skip_values(int type, uint32_t count)
+    {
+        size_t bytes = (type_size(type) * count + 3) & ~(size_t)3;
+        if (o + bytes > n) { ok = false; return; }
+        o += bytes;
+    }
+};
+
+double be_f64(const unsigned char* q)
+{
+    uint64_t v = 0;
+    for (int i = 0; i < 8; ++i) v = (v << 8) | q[i];
+    double d;
+    memcpy(&d, &v, 8);
+    return d;
+}
+float be_f32(const unsigned char* q)
+{
+    uint32_t v = ((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3];
+    float f;
+    memcpy(&f, &v, 4);
+    return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const fields[8], int iprec)
+{
+    if (!path || !meta || !fields || (iprec != 4 && iprec != 8)) return MHD_E_ARG;
+    const int nx = meta->nx, ny = meta->ny, nz = meta->nz;
+    const size_t n3 = (size_t)nx * ny * nz;
+    for (int f = 0; f < 8; ++f)
+        if (!fields[f]) return MHD_E_ARG;
+    Out h;
+    h.b.insert(h.b.end(), {'C', 'D', 'F', 1});
+    h.u32(0);  // numrecs (no record dimension)
+    // dimensions
+    h.u32(NC_DIMENSION);
+    h.u32(4);
+    const char* dn[4] = {"nx", "ny", "nz", "one"};
+    const int dl[4] = {nx, ny, nz, 1};
+    for (int i = 0; i < 4; ++i) {
+        h.name(dn[i]);
+        h.u32((uint32_t)dl[i]);
+    }
+    // global attributes
+    const double* pv = &meta->gamma;
+    h.u32(NC_ATTRIBUTE);
+    h.u32(13);
+    h.name("version");
+    h.u32(NC_INT);
+    h.u32(1);
+    h.i32(meta->version);
+    std::string hist(meta->history);
+    h.name("history");
+    h.u32(NC_CHAR);
+    h.u32((uint32_t)hist.size());
+    h.b.insert(h.b.end(), hist.begin(), hist.end());
+    h.pad4();
+    for (int i = 0; i < 11; ++i) {
+        h.name(kAttrs[i]);
+        h.u32(NC_DOUBLE);
+        h.u32(1);
+        h.f64(pv[i]);
+    }
+    // variables: sizes and offsets follow the header, so lay the header out
+    // with placeholder offsets first and patch them
+    const int ftype = iprec == 8 ? NC_DOUBLE : NC_FLOAT;
+    const size_t fsize = (size_t)iprec * n3;
+    struct V { const char* name; int ndim; int dims[3]; int type; size_t vsize; bool units; };
+    std::vector<V> vars = {{"time", 1, {3, 0, 0}, NC_DOUBLE, 8, true},
+                           {"iteration", 1, {3, 0, 0}, NC_INT, 4, false},
+                           {"timestep", 1, {3, 0, 0}, NC_DOUBLE, 8, false}};
+    for (int f = 0; f < 8; ++f) vars.push_back({kFields[f], 3, {2, 1, 0}, ftype, (fsize + 3) & ~(size_t)3, false});
+    if (fsize > 0x7fffffffULL) return MHD_E_ARG;  // CDF-1 variable size limit
+    h.u32(NC_VARIABLE);
+    h.u32((uint32_t)vars.size());
+    std::vector<size_t> begin_pos;
+    for (const V& v : vars) {
+        h.name(v.name);
+        h.u32((uint32_t)v.ndim);
+        for (int d = 0; d < v.ndim; ++d) h.u32((uint32_t)v.dims[d]);
+        if (v.units) {
+            h.u32(NC_ATTRIBUTE);
+            h.u32(1);
+            h.name("units");
+            h.u32(NC_CHAR);
+            h.u32(7);
+            h.b.insert(h.b.end(), {'U', 'N', 'K', 'N', 'O', 'W', 'N'});
+            h.pad4();
+        } else {
+            h.u32(0);
+            h.u32(0);
+        }
+        h.u32((uint32_t)v.type);
+        h.u32((uint32_t)(v.vsize > 0xffffffffULL ? 0xffffffffULL : v.vsize));
+        begin_pos.push_back(h.b.size());
+        h.u32(0);
+    }
+    size_t off = h.b.size();
+    for (size_t i = 0; i < vars.size(); ++i) {
+        if (off > 0xffffffffULL) return MHD_E_ARG;  // beyond 32-bit offsets
+        const uint32_t b = (uint32_t)off;
+        for (int s = 0; s < 4; ++s) h.b[begin_pos[i] + s] = (unsigned char)(b >> (24 - 8 * s));
+        off += vars[i].vsize;
+    }
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return -13;
+    bool ok = fwrite(h.b.data(), 1, h.b.size(), fp) == h.b.size();
+    Out d;
+    d.f64(meta->time);
+    d.i32(meta->iteration);
+    d.f64(meta->timestep);
+    ok = ok && fwrite(d.b.data(), 1, d.b.size(), fp) == d.b.size();
+    std::vector<unsigned char> buf(fsize + 4);
+    for (int f = 0; f < 8 && ok; ++f) {
+        const double* src = fields[f];
+        if (iprec == 8) {
+            for (size_t i = 0; i < n3; ++i) {
+                uint64_t v;
+                memcpy(&v, &src[i], 8);
+                for (int s = 0; s < 8; ++s) buf[8 * i + s] = (unsigned char)(v >> (56 - 8 * s));
+            }
+        } else {
+            for (size_t i = 0; i < n3; ++i) {
+                const float x = (float)src[i];
+                uint32_t v;
+                memcpy(&v, &x, 4);
+                for (int s = 0; s < 4; ++s) buf[4 * i + s] = (unsigned char)(v >> (24 - 8 * s));
+            }
+        }
+        const size_t padded = (fsize + 3) & ~(size_t)3;
+        for (size_t i = fsize; i < padded; ++i) buf[i] = 0;
+        ok = fwrite(buf.data(), 1, padded, fp) == padded;
+    }
+    ok = (fclose(fp) == 0) && ok;
+    return ok ? MHD_OK : -14;
+}
+
+static int read_file(const char* path, std::vector<unsigned char>& data)
+{
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return -13;
+    fseek(fp, 0, SEEK_END);
+    long sz = ftell(fp);
+    fseek(fp, 0, SEEK_SET);
+    if (sz < 8) { fclose(fp); return -15; }
+    data.resize((size_t)sz);
+    size_t got = fread(data.data(), 1, (size_t)sz, fp);
+    fclose(fp);
+    return got == (size_t)sz ? 0 : -15;
+}
+
+// fields may be NULL to query meta only (dimensions, attributes, scalars)
+int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8])
+{
+    if (!path || !meta) return MHD_E_ARG;
+    std::vector<unsigned char> data;
+    int e = read_file(path, data);
+    if (e) return e;
+    In in{data.data(), data.size()};
+    if (memcmp(data.data(), "CDF\x01", 4) != 0) return -15;
+    in.o = 4;
+    in.u32();  // numrecs
+    std::vector<std::string> dname;
+    std::vector<uint32_t> dlen;
+    uint32_t tag = in.u32(), cnt = in.u32();
+    if (tag == NC_DIMENSION)
+        for (uint32_t i = 0; i < cnt && in.ok; ++i) {
+            dname.push_back(in.name());
+            dlen.push_back(in.u32());
+        }
+    memset(meta, 0, sizeof(*meta));
+    for (size_t i = 0; i < dname.size(); ++i) {
+        if (dname[i] == "nx") meta->nx = (int)dlen[i];
+        if (dname[i] == "ny") meta->ny = (int)dlen[i];
+        if (dname[i] == "nz") meta->nz = (int)dlen[i];
+    }
+    double* pv = &meta->gamma;
+    tag = in.u32();
+    cnt = in.u32();
+    if (tag == NC_ATTRIBUTE)
+        for (uint32_t i = 0; i < cnt && in.ok; ++i) {
+            std::string nm = in.name();
+            int type = (int)in.u32();
+            uint32_t count = in.u32();
+            const unsigned char* q = data.data() + in.o;
+            if (nm == "version" && type == NC_INT && count == 1)
+                meta->version = (int)(((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3]);
+            if (nm == "history" && type == NC_CHAR) {
+                size_t L = count < sizeof(meta->history) - 1 ? count : sizeof(meta->history) - 1;
+                memcpy(meta->history, q, L);
+                meta->history[L] = 0;
+            }
+            for (int a = 0; a < 11; ++a)
+                if (nm == kAttrs[a] && type == NC_DOUBLE && count >= 1) pv[a] = be_f64(q);
+            in.skip_values(type, count);
+        }
+    tag = in.u32();
+    cnt = in.u32();
+    if (!in.ok || tag != NC_VARIABLE) return -15;
+    const size_t n3 = (size_t)meta->nx * meta->ny * meta->nz;
+    int found = 0;
+    for (uint32_t i = 0; i < cnt && in.ok; ++i) {
+        std::string nm = in.name();
+        uint32_t nd = in.u32();
+        for (uint32_t d = 0; d < nd; ++d) in.u32();
+        uint32_t at = in.u32(), ac = in.u32();
+        if (at == NC_ATTRIBUTE)
+            for (uint32_t a = 0; a < ac && in.ok; ++a) {
+                in.name();
+                int type = (int)in.u32();
+                uint32_t count = in.u32();
+                in.skip_values(type, count);
+            }
+        int type = (int)in.u32();
+        in.u32();  // vsize
+        size_t begin = in.u32();
+        if (!in.ok || begin >= data.size()) return -15;
+        const unsigned char* q = data.data() + begin;
+        if (nm == "time" && type == NC_DOUBLE) meta->time = be_f64(q);
+        if (nm == "timestep" && type == NC_DOUBLE) meta->timestep = be_f64(q);
+        if (nm == "iteration" && type == NC_INT) meta->iteration = (int)(((uint32_t)q[0] << 24) | (q[1] << 16) | (q[2] << 8) | q[3]);
+        for (int f = 0; f < 8; ++f) {
+            if (nm != kFields[f]) continue;
+            const size_t need = n3 * type_size(type);
+            if ((type != NC_DOUBLE && type != NC_FLOAT) || begin + need > data.size()) return -15;
+            ++found;
+            if (fields && fields[f]) {
+                double* dst = fields[f];
+                if (type == NC_DOUBLE)
+                    for (size_t k = 0; k < n3; ++k) dst[k] = be_f64(q + 8 * k);
+                else
+                    for (size_t k = 0; k < n3; ++k) dst[k] = (double)be_f32(q + 4 * k);
+            }
+        }
+    }
+    if (!in.ok) return -15;
+    return found == 8 ? MHD_OK : -15;
+}
+
+}  // extern "C"

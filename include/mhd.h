@@ -155,6 +155,26 @@ enum { MHD_K_NONE = 0, MHD_K_XDERIV2 = 1, MHD_K_YDERIV2 = 2, MHD_K_ZDERIV = 3, M
 int mhd_kernel_timing(mhd_ctx_t* ctx, int kernel, int enable);
 int mhd_kernel_time(mhd_ctx_t* ctx, double* total_ms, long* launches);
 
+/* Dump / restart files in the reference's layout (CreateNetCDF, WriteNetCDF,
+ * ReadNetCDF: src/netcdf.F90:39-1287): netCDF classic (CDF-1), dimensions
+ * nx, ny, nz, one; global attributes version, history, gamma .. ymax;
+ * variables time, iteration, timestep and R, T, U, V, W, Bx, By, Bz over
+ * (nz, ny, nx), double (iprec 8) or float (iprec 4).  Host-only, no device
+ * calls, no libnetcdf.  fields[] = {R, T, U, V, W, Bx, By, Bz}; the reader
+ * accepts NULL fields to query meta only, and converts float files to double.
+ * Errors: -13 cannot open/create (abort_comms(13)), -14 write failed,
+ * -15 not a readable classic file with those variables. */
+typedef struct mhd_dump {
+    int nx, ny, nz;
+    int version;
+    int iteration;
+    double time, timestep;
+    double gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax;
+    char history[256];
+} mhd_dump_t;
+int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const fields[8], int iprec);
+int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8]);
+
 /* Waits for all work of the context; returns any pending device error. */
 int mhd_sync(mhd_ctx_t* ctx);
 /* Bytes of device memory held by the context. */

@@ -1,0 +1,170 @@
+"""Dump / restart files (src/netcdf.F90:39-580 CreateNetCDF, WriteNetCDF,
+ReadNetCDF): the library writes netCDF classic files without libnetcdf.
+
+Pinned three ways: scipy's independent netCDF-3 reader/writer reads what the
+library writes and writes what the library reads, and the structure of the
+reference's own restart fixture (bin/restart.nc, captured as
+tests/golden/restart_layout.json) must be reproduced exactly: dimension
+order and sizes, attribute names/order/types, variable names, dimensions,
+types and the time variable's units attribute.  No GPU is used.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+import fug30v0kxc_amd as fa
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LAYOUT = json.load(open(os.path.join(HERE, "golden", "restart_layout.json")))
+NAMES = {"r": "R", "t": "T", "u": "U", "v": "V", "w": "W", "bx": "Bx", "by": "By", "bz": "Bz"}
+
+
+def _fields(shape, seed=0):
+    rng = np.random.default_rng(seed)
+    return {n: rng.standard_normal(shape) for n in fa.DUMP_FIELDS}
+
+
+def _params():
+    return dict(gamma=1.4, sigma=1.0, theta=10.0, ck=0.0717, cm=1.0, f=2.57, tayl=0.0,
+                psi=1.570796327, zeta=0.2, xmax=4.0, ymax=4.0)
+
+
+@pytest.mark.parametrize("iprec", [8, 4])
+def test_written_file_read_by_scipy(tmp_path, iprec):
+    shape = (6, 5, 4)      # nz, ny, nx (odd sizes catch an x/y swap)
+    flds = _fields(shape)
+    path = tmp_path / "T16_001.nc"
+    fa.write_dump(path, flds, iteration=1234, time=1.5, timestep=2.5e-4, params=_params(),
+                  iprec=iprec, history="test dump")
+    f = netcdf_file(str(path), "r", mmap=False)
+    assert f.version_byte == 1
+    assert dict(f.dimensions) == {"nx": 4, "ny": 5, "nz": 6, "one": 1}
+    assert f.history == b"test dump"
+    assert int(np.atleast_1d(f.version)[0]) == 3
+    for k, v in _params().items():
+        assert float(np.atleast_1d(getattr(f, k))[0]) == v
+    assert float(f.variables["time"].data[0]) == 1.5
+    assert int(f.variables["iteration"].data[0]) == 1234
+    assert float(f.variables["timestep"].data[0]) == 2.5e-4
+    for n, nc in NAMES.items():
+        var = f.variables[nc]
+        assert var.dimensions == ("nz", "ny", "nx")
+        assert var.typecode() == ("d" if iprec == 8 else "f")
+        want = flds[n] if iprec == 8 else flds[n].astype(np.float32)
+        np.testing.assert_array_equal(np.asarray(var.data), want)
+    f.close()
+
+
+@pytest.mark.parametrize("typecode", ["d", "f"])
+def test_read_file_written_by_scipy(tmp_path, typecode):
+    shape = (7, 3, 8)
+    flds = _fields(shape, seed=3)
+    path = tmp_path / "restart.nc"
+    f = netcdf_file(str(path), "w", version=1)
+    f.createDimension("one", 1)           # different dimension order than the writer's
+    f.createDimension("nz", shape[0])
+    f.createDimension("ny", shape[1])
+    f.createDimension("nx", shape[2])
+    f.history = b"scipy"
+    for k, v in _params().items():
+        setattr(f, k, np.float64(v))
+    f.version = np.int32(3)
+    for n, nc in reversed(list(NAMES.items())):
+        var = f.createVariable(nc, typecode, ("nz", "ny", "nx"))
+        var[:] = flds[n]
+    t = f.createVariable("time", "d", ("one",))
+    t[:] = 9.25
+    it = f.createVariable("iteration", "i", ("one",))
+    it[:] = 777
+    ts = f.createVariable("timestep", "d", ("one",))
+    ts[:] = 1e-3
+    f.close()
+    got, meta = fa.read_dump(path)
+    assert (meta["nx"], meta["ny"], meta["nz"]) == (8, 3, 7)
+    assert meta["iteration"] == 777 and meta["time"] == 9.25 and meta["timestep"] == 1e-3
+    assert meta["version"] == 3 and meta["history"] == "scipy"
+    for k, v in _params().items():
+        assert meta[k] == v
+    for n in fa.DUMP_FIELDS:
+        want = flds[n] if typecode == "d" else flds[n].astype(np.float32).astype(np.float64)
+        np.testing.assert_array_equal(got[n], want)
+
+
+def test_round_trip_through_library(tmp_path):
+    shape = (4, 6, 2)
+    flds = _fields(shape, seed=5)
+    path = tmp_path / "a.nc"
+    fa.write_dump(path, flds, iteration=5, time=0.125, timestep=1e-4, params=_params())
+    got, meta = fa.read_dump(path)
+    for n in fa.DUMP_FIELDS:
+        np.testing.assert_array_equal(got[n], flds[n])
+    assert meta["iteration"] == 5
+
+
+def test_layout_matches_reference_restart(tmp_path):
+    """Same structure as the reference's bin/restart.nc (30x30x60)."""
+    dims = LAYOUT["dimensions"]
+    shape = (dims["nz"], dims["ny"], dims["nx"])
+    path = tmp_path / "restart.nc"
+    attrs = LAYOUT["global_attributes"]
+    params = {k: attrs[k]["value"][0] for k in _params()}
+    scal = {v["name"]: v["value"][0] for v in LAYOUT["variables"] if "value" in v}
+    fa.write_dump(path, _fields(shape), iteration=int(scal["iteration"]), time=scal["time"],
+                  timestep=scal["timestep"], params=params, iprec=8,
+                  history=attrs["history"]["value"])
+    f = netcdf_file(str(path), "r", mmap=False)
+    assert f.version_byte == LAYOUT["version_byte"]
+    assert list(f.dimensions.keys()) == list(dims.keys())
+    assert dict(f.dimensions) == dims
+    assert list(f._attributes.keys()) == LAYOUT["attribute_order"]
+    for k, a in attrs.items():
+        v = f._attributes[k]
+        if a["type"] == "char":
+            assert v.decode() == a["value"]
+        else:
+            assert str(np.atleast_1d(v).dtype) == a["type"]
+            assert np.atleast_1d(v).tolist() == a["value"]
+    assert list(f.variables.keys()) == [v["name"] for v in LAYOUT["variables"]]
+    for v in LAYOUT["variables"]:
+        var = f.variables[v["name"]]
+        assert list(var.dimensions) == v["dims"]
+        assert var.typecode() == v["typecode"]
+        got_attrs = {k: (x.decode() if isinstance(x, bytes) else x)
+                     for k, x in var._attributes.items()}
+        assert got_attrs == v["attrs"]
+        if "value" in v:
+            assert np.asarray(var.data).tolist() == v["value"]
+    f.close()
+
+
+def test_reference_restart_statistics_survive_round_trip(tmp_path):
+    """A field with the fixture's extreme values is stored bit-exactly."""
+    s = next(v for v in LAYOUT["variables"] if v["name"] == "T")["stats"]
+    flds = _fields((3, 4, 4))
+    flds["t"][0, 0, 0], flds["t"][2, 3, 3] = s["min"], s["max"]
+    path = tmp_path / "s.nc"
+    fa.write_dump(path, flds)
+    got, _ = fa.read_dump(path)
+    assert got["t"][0, 0, 0] == s["min"] and got["t"][2, 3, 3] == s["max"]
+
+
+def test_errors(tmp_path):
+    with pytest.raises(fa.MhdError):
+        fa.read_dump(tmp_path / "missing.nc")
+    bad = tmp_path / "bad.nc"
+    bad.write_bytes(b"not a netcdf file at all")
+    with pytest.raises(fa.MhdError):
+        fa.read_dump(bad)
+    # a classic file without the solver's variables
+    f = netcdf_file(str(tmp_path / "other.nc"), "w", version=1)
+    f.createDimension("x", 3)
+    v = f.createVariable("q", "d", ("x",))
+    v[:] = 1.0
+    f.close()
+    with pytest.raises(fa.MhdError):
+        fa.read_dump(tmp_path / "other.nc")
+    with pytest.raises(fa.MhdError):
+        fa.write_dump(tmp_path / "x.nc", _fields((2, 2, 2)), iprec=3)
