@@ -37,7 +37,7 @@ OPS = {"d1bydx": 1, "d1bydy": 2, "d2bydx": 3, "d2bydy": 4, "d1bydz": 5, "d1upbyd
        "d2bydz": 7, "dealias": 8, "forwardfft": 9, "inversefft": 10, "d1bydz1D": 11,
        "d2bydz1D": 12}
 
-KERNELS = {"none": 0, "xderiv2": 1, "yderiv2": 2, "zderiv": 3, "hydro_a": 4, "ab3": 5}
+KERNELS = {"none": 0, "xderiv2": 1, "yderiv2": 2, "zderiv": 3, "hydro": 4, "ab3": 5}
 
 
 class MhdParams(ctypes.Structure):

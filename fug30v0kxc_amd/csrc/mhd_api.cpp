@@ -36,8 +36,7 @@ namespace {
 enum Wk {
     W_RX, W_TX, W_TXX, W_UX, W_UXX, W_VX, W_VXX, W_WX, W_WXX, W_XBB, W_XZX,
     W_RY, W_TY, W_TYY, W_UY, W_UYY, W_VY, W_VYY, W_WY, W_WYY, W_YBB, W_YZY,
-    W_TZU, W_TZC, W_TZZ, W_WZU, W_WZC, W_UZU, W_UZZ, W_VZU, W_VZZ, W_ZZX, W_ZZY, W_RZU, W_RZC,
-    W_C1, W_C2, W_C3, W_C4, W_DXC1, W_DYC2, W_DZC3, W_DZC4,
+    W_DXM, W_DYM, W_UXY, W_VXY,
     W_COUNT
 };
 
@@ -70,6 +69,7 @@ struct mhd_ctx {
     double *red_partial, *red_result, *d_bzbar;
     double* stage;     // device FP64 staging (one field)
     int max_groups;    // resident workgroups for the persistent transform kernels
+    int hzc;           // planes per thread of the column-walking hydro kernel
     double* hbuf;      // pinned host staging (one field)
 
     mhd_scalars_t sc;
@@ -285,9 +285,9 @@ int zrun(mhd_ctx* c, const ZLaunch& L)
     return e == 7 ? MHD_E_FLAG : (e ? MHD_E_DEVICE : 0);
 }
 
-HydroArgs hargs(mhd_ctx* c)
+HydroColArgs colargs(mhd_ctx* c)
 {
-    HydroArgs a;
+    HydroColArgs a;
     memset(&a, 0, sizeof(a));
     a.g = c->g;
     a.p = c->ph;
@@ -295,21 +295,18 @@ HydroArgs hargs(mhd_ctx* c)
     a.bx = c->bx; a.by = c->by; a.bz = c->bz; a.jsq = c->jsq;
     a.kappa = c->kappa; a.dkappa = c->dkappa;
     void** W = c->wk;
-    a.rx = W[W_RX]; a.tx = W[W_TX]; a.txx = W[W_TXX]; a.ux = W[W_UX]; a.uxx = W[W_UXX];
+    a.rx = W[W_RX]; a.tx = W[W_TX]; a.txx = W[W_TXX]; a.ux = W[W_UX]; a.uxxc = W[W_UXX];
     a.vx = W[W_VX]; a.vxx = W[W_VXX]; a.wx = W[W_WX]; a.wxx = W[W_WXX]; a.xbb = W[W_XBB]; a.xzx = W[W_XZX];
+    a.dxm = W[W_DXM];
     a.ry = W[W_RY]; a.ty = W[W_TY]; a.tyy = W[W_TYY]; a.uy = W[W_UY]; a.uyy = W[W_UYY];
-    a.vy = W[W_VY]; a.vyy = W[W_VYY]; a.wy = W[W_WY]; a.wyy = W[W_WYY]; a.ybb = W[W_YBB]; a.yzy = W[W_YZY];
-    a.tzu = W[W_TZU]; a.tzc = W[W_TZC]; a.tzz = W[W_TZZ]; a.wzu = W[W_WZU]; a.wzc = W[W_WZC];
-    a.uzu = W[W_UZU]; a.uzz = W[W_UZZ]; a.vzu = W[W_VZU]; a.vzz = W[W_VZZ]; a.zzx = W[W_ZZX]; a.zzy = W[W_ZZY];
-    a.rzu = W[W_RZU]; a.rzc = W[W_RZC];
-    a.dxc1 = W[W_DXC1]; a.dyc2 = W[W_DYC2]; a.dzc3 = W[W_DZC3]; a.dzc4 = W[W_DZC4];
-    a.c1 = W[W_C1]; a.c2 = W[W_C2]; a.c3 = W[W_C3]; a.c4 = W[W_C4];
+    a.vy = W[W_VY]; a.vyyc = W[W_VYY]; a.wy = W[W_WY]; a.wyy = W[W_WYY]; a.ybb = W[W_YBB]; a.yzy = W[W_YZY];
+    a.dym = W[W_DYM]; a.uxy = W[W_UXY]; a.vxy = W[W_VXY];
     a.rdot = level_ptr(c, 3, 3);
     a.tdot = level_ptr(c, 4, 3);
     a.udot = level_ptr(c, 0, 3);
     a.vdot = level_ptr(c, 1, 3);
     a.wdot = level_ptr(c, 2, 3);
-    a.wf32 = c->wf32;
+    a.zc = c->hzc;
     return a;
 }
 
@@ -353,13 +350,16 @@ int run_hydro_deriv(mhd_ctx* c)
 {
     const int wf = c->wf32;
     void** W = c->wk;
+    const Phys& P = c->ph;
     const double f = c->p.f;
-    // x derivatives: first+second of t,u,v,w; first of r and of the two
-    // Maxwell-stress products F*by*bx, F*bz*bx
+    // x pass: d/dx, d2/dx2 of t, u, v, w (u's second output also carries the
+    // scko3 d/dx d/dx u of the composite flux, lines 181, 258); d/dx of r and
+    // of the Maxwell-stress products F by bx, F bz bx, F/2 (bx^2-by^2-bz^2)
     {
         SpecLaunch L = spec_base(c, 0, SM_DERIV2);
         L.job[0] = sjob(c->t, 0, W[W_TX], W[W_TXX], MUL_D1, MUL_D2, wf);
-        L.job[1] = sjob(c->u, 0, W[W_UX], W[W_UXX], MUL_D1, MUL_D2, wf);
+        L.job[1] = sjob(c->u, 0, W[W_UX], W[W_UXX], MUL_D1, MUL_D2C, wf);
+        L.job[1].mc[0] = P.sck; L.job[1].mc[1] = P.scko3;
         L.job[2] = sjob(c->v, 0, W[W_VX], W[W_VXX], MUL_D1, MUL_D2, wf);
         L.job[3] = sjob(c->w, 0, W[W_WX], W[W_WXX], MUL_D1, MUL_D2, wf);
         L.njobs = 4;
@@ -370,68 +370,37 @@ int run_hydro_deriv(mhd_ctx* c)
         M.job[1].load_op = LD_PROD; M.job[1].in[1] = c->bx; M.job[1].scale = f;
         M.job[2] = sjob(c->bz, wf, W[W_XZX], nullptr, MUL_D1, 0, wf);
         M.job[2].load_op = LD_PROD; M.job[2].in[1] = c->bx; M.job[2].scale = f;
-        M.njobs = 3;
+        M.job[3] = sjob(c->bx, wf, W[W_DXM], nullptr, MUL_D1, 0, wf);
+        M.job[3].load_op = LD_MAXW; M.job[3].in[1] = c->by; M.job[3].in[2] = c->bz; M.job[3].scale = P.fo2;
+        M.njobs = 4;
         TRY(spec(c, M));
     }
-    // y derivatives
+    // y pass: the same along y (v's second output carries scko3 d/dy d/dy v),
+    // plus the mixed derivatives d/dy(ux), d/dy(vx) of the composite fluxes
     {
         SpecLaunch L = spec_base(c, 1, SM_DERIV2);
         L.job[0] = sjob(c->t, 0, W[W_TY], W[W_TYY], MUL_D1, MUL_D2, wf);
         L.job[1] = sjob(c->u, 0, W[W_UY], W[W_UYY], MUL_D1, MUL_D2, wf);
-        L.job[2] = sjob(c->v, 0, W[W_VY], W[W_VYY], MUL_D1, MUL_D2, wf);
+        L.job[2] = sjob(c->v, 0, W[W_VY], W[W_VYY], MUL_D1, MUL_D2C, wf);
+        L.job[2].mc[0] = P.sck; L.job[2].mc[1] = P.scko3;
         L.job[3] = sjob(c->w, 0, W[W_WY], W[W_WYY], MUL_D1, MUL_D2, wf);
         L.njobs = 4;
         TRY(spec(c, L, MHD_K_YDERIV2));
         SpecLaunch M = spec_base(c, 1, SM_DERIV1);
         M.job[0] = sjob(c->r, 0, W[W_RY], nullptr, MUL_D1, 0, wf);
-        M.job[1] = sjob(c->by, wf, W[W_YBB], nullptr, MUL_D1, 0, wf);
-        M.job[1].load_op = LD_PROD; M.job[1].in[1] = c->bx; M.job[1].scale = f;
-        M.job[2] = sjob(c->bz, wf, W[W_YZY], nullptr, MUL_D1, 0, wf);
-        M.job[2].load_op = LD_PROD; M.job[2].in[1] = c->by; M.job[2].scale = f;
-        M.njobs = 3;
+        M.job[1] = sjob(W[W_UX], wf, W[W_UXY], nullptr, MUL_D1, 0, wf);
+        M.job[2] = sjob(W[W_VX], wf, W[W_VXY], nullptr, MUL_D1, 0, wf);
+        M.job[3] = sjob(c->by, wf, W[W_YBB], nullptr, MUL_D1, 0, wf);
+        M.job[3].load_op = LD_PROD; M.job[3].in[1] = c->bx; M.job[3].scale = f;
+        M.job[4] = sjob(c->bz, wf, W[W_YZY], nullptr, MUL_D1, 0, wf);
+        M.job[4].load_op = LD_PROD; M.job[4].in[1] = c->by; M.job[4].scale = f;
+        M.job[5] = sjob(c->by, wf, W[W_DYM], nullptr, MUL_D1, 0, wf);
+        M.job[5].load_op = LD_MAXW; M.job[5].in[1] = c->bx; M.job[5].in[2] = c->bz; M.job[5].scale = P.fo2;
+        M.njobs = 6;
         TRY(spec(c, M));
     }
-    // z derivatives (flags as in the reference's calls)
-    {
-        ZLaunch L = zbase(c);
-        int n = 0;
-        L.job[n++] = zjob(c->t, 0, W[W_TZU], wf, ZK_D1UP, 5);  // 66
-        L.job[n++] = zjob(c->t, 0, W[W_TZC], wf, ZK_D1, 5);    // 67
-        L.job[n++] = zjob(c->t, 0, W[W_TZZ], wf, ZK_D2, 0);    // 68
-        L.job[n++] = zjob(c->w, 0, W[W_WZU], wf, ZK_D1UP, 1);  // 124
-        L.job[n++] = zjob(c->w, 0, W[W_WZC], wf, ZK_D1, 1);    // 125
-        L.job[n++] = zjob(c->u, 0, W[W_UZU], wf, ZK_D1UP, 2);  // 233
-        L.job[n++] = zjob(c->u, 0, W[W_UZZ], wf, ZK_D2, 3);    // 349
-        L.job[n++] = zjob(c->v, 0, W[W_VZU], wf, ZK_D1UP, 2);  // 275
-        L.job[n++] = zjob(c->v, 0, W[W_VZZ], wf, ZK_D2, 3);    // 350
-        L.job[n] = zjob(c->bz, wf, W[W_ZZX], wf, ZK_D1, 5);    // 272: F*bz*bx
-        L.job[n].load_op = LD_PROD; L.job[n].in1 = c->bx; L.job[n].scale = f; ++n;
-        L.job[n] = zjob(c->bz, wf, W[W_ZZY], wf, ZK_D1, 5);    // 321: F*bz*by
-        L.job[n].load_op = LD_PROD; L.job[n].in1 = c->by; L.job[n].scale = f; ++n;
-        L.job[n++] = zjob(c->r, 0, W[W_RZU], wf, ZK_D1UP, 0);  // 443
-        L.job[n++] = zjob(c->r, 0, W[W_RZC], wf, ZK_D1, 0);    // 444
-        L.njobs = n;
-        TRY(zrun(c, L));
-    }
-    HydroArgs a = hargs(c);
-    TRY(timed(c, MHD_K_HYDRO_A, [&] { return kerr(launch_hydro_a(a, c->s)); }));
-    // derivatives of the composite fluxes (181-184)
-    {
-        SpecLaunch L = spec_base(c, 0, SM_DERIV1);
-        L.job[0] = sjob(W[W_C1], wf, W[W_DXC1], nullptr, MUL_D1, 0, wf);
-        L.njobs = 1;
-        TRY(spec(c, L));
-        SpecLaunch M = spec_base(c, 1, SM_DERIV1);
-        M.job[0] = sjob(W[W_C2], wf, W[W_DYC2], nullptr, MUL_D1, 0, wf);
-        M.njobs = 1;
-        TRY(spec(c, M));
-        ZLaunch Z = zbase(c);
-        Z.job[0] = zjob(W[W_C3], wf, W[W_DZC3], wf, ZK_D1, 2);
-        Z.job[1] = zjob(W[W_C4], wf, W[W_DZC4], wf, ZK_D1, 5);
-        Z.njobs = 2;
-        TRY(zrun(c, Z));
-    }
-    return kerr(launch_hydro_b(a, c->s));
+    HydroColArgs a = colargs(c);
+    return timed(c, MHD_K_HYDRO, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); });
 }
 
 // ---------------------------------------------------------------------------
@@ -763,6 +732,15 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->ring[0] = 0; c->ring[1] = 1; c->ring[2] = 2;
     c->sc.first = 1;
     c->max_groups = 8 * ncu;   // enough resident workgroups to keep every CU busy
+    {
+        // column-walk chunks: about 2048 workgroups of 256 columns, >= 8 planes each
+        const long bx = (c->nxy + 255) / 256;
+        long nch = 2048 / (bx > 0 ? bx : 1);
+        const long nin = c->nz > 8 ? c->nz - 8 : 1;   // interior planes
+        if (nch > nin / 8) nch = nin / 8;
+        if (nch < 1) nch = 1;
+        c->hzc = (int)((nin + nch - 1) / nch);
+    }
 
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return MHD_E_DEVICE; }
 
@@ -1171,9 +1149,9 @@ int mhd_op(mhd_ctx_t* c, int op, const double* in, double* out, double* out2, in
     if (!in || !out) return MHD_E_ARG;
     const int wf = c->wf32;
     const long n3 = c->n3;
-    void* A = c->wk[W_C1];
-    void* B = c->wk[W_C2];
-    void* C = c->wk[W_C3];
+    void* A = c->wk[W_DXM];
+    void* B = c->wk[W_DYM];
+    void* C = c->wk[W_UXY];
     if (op == MHD_OP_D1BYDZ1D || op == MHD_OP_D2BYDZ1D) {
         // nz-vectors: run the 3-D kernel on a 1x1 column broadcast over the plane
         std::vector<double> col(n3);

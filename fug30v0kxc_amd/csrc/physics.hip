@@ -144,135 +144,6 @@ int launch_z(const ZLaunch& L, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// hydro RHS, src/hydro_deriv.f90.  Stage A holds every term that needs only
-// primitive-field derivatives, in the reference's order of accumulation
-// (lines 45-432) except the three composite-flux derivatives, which stage B
-// adds once they exist, followed by the boundary d(rho)/dz substitution
-// (455-482) and the division by rho (487-497).
-// ---------------------------------------------------------------------------
-template <typename TW>
-__global__ void __launch_bounds__(NT) hydro_a_kernel(HydroArgs a)
-{
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
-    if (q >= a.g.n3) return;
-    const int k = (int)(q / a.g.nxy);
-    const Phys& p = a.p;
-    auto W = [&](const void* ptr) -> double { return (double)((const TW*)ptr)[q]; };
-    const double rr = a.r[q], tt = a.t[q], u0 = a.u[q], v0 = a.v[q], w0 = a.w[q];
-    const double kap = a.kappa[k], dkap = a.dkappa[k];
-    // 45-53, 82-110
-    const double rx = W(a.rx), ry = W(a.ry), tx = W(a.tx), ty = W(a.ty), tzu = W(a.tzu);
-    double rdot = -u0 * rx - v0 * ry;
-    const double exv = -tt * rx, eyv = -tt * ry;
-    double tdot = -rr * (u0 * tx + v0 * ty + w0 * tzu) + p.gck * kap * (W(a.txx) + W(a.tyy) + W(a.tzz)) +
-                  p.gck * dkap * tzu + p.fzckg * W(a.jsq);
-    double udot = exv - rr * tx;
-    double vdot = eyv - rr * ty;
-    double wdot = -rr * W(a.tzc);
-    // 139-168
-    const double z1 = W(a.ux), z2 = W(a.vy), z3 = W(a.wzu), z4 = W(a.wzc);
-    double divu = z1 + z2 + z4;
-    rdot = rdot - rr * divu;
-    tdot = tdot - p.gm1 * tt * divu * rr + 2 * p.sckg * (z1 * z1 + z2 * z2 + z4 * z4 - (1.0 / 3.0) * divu * divu);
-    const double bxv = W(a.bx), byv = W(a.by), bzv = W(a.bz);
-    const double temp1 = p.fo2 * bxv * bxv, temp2 = p.fo2 * byv * byv, temp3 = p.fo2 * bzv * bzv;
-    divu = p.scko3 * divu;
-    udot = udot - rr * u0 * z1;
-    vdot = vdot - rr * v0 * z2;
-    wdot = wdot - rr * w0 * z3;
-    ((TW*)a.c1)[q] = (TW)(divu + temp1 - temp2 - temp3);
-    ((TW*)a.c2)[q] = (TW)(divu + temp2 - temp1 - temp3);
-    ((TW*)a.c3)[q] = (TW)(p.scko3 * z1 + p.scko3 * z2 + temp3 - temp1 - temp2);
-    ((TW*)a.c4)[q] = (TW)(4.0 * p.scko3 * z4);
-    // 201-219 (the d/dx, d/dy, d/dz of c1..c4 follow in stage B)
-    {
-        const double w1 = W(a.uy), w2 = W(a.vx);
-        udot = udot - rr * v0 * w1;
-        vdot = vdot - rr * u0 * w2;
-        tdot = tdot + p.sckg * (w1 + w2) * (w1 + w2);
-    }
-    // 245-261
-    udot = udot + W(a.ybb);
-    vdot = vdot + W(a.xbb);
-    {
-        const double w3 = W(a.wx), w4 = W(a.uzu);
-        udot = udot - rr * w0 * w4;
-        wdot = wdot - rr * u0 * w3;
-        tdot = tdot + p.sckg * (w3 + w4) * (w3 + w4);
-    }
-    // 287-309
-    udot = udot + W(a.zzx);
-    wdot = wdot + W(a.xzx) + p.thmplus1 * rr;
-    {
-        const double w1 = W(a.wy), w2 = W(a.vzu);
-        vdot = vdot - rr * w0 * w2;
-        wdot = wdot - rr * v0 * w1;
-        tdot = tdot + p.sckg * (w1 + w2) * (w1 + w2);
-    }
-    udot = udot - rr * p.cor * (w0 * p.cpsi + v0 * p.spsi);
-    vdot = vdot - rr * p.cor * (-u0 * p.spsi);
-    wdot = wdot - rr * p.cor * (-u0 * p.cpsi);
-    // 331-338
-    vdot = vdot + W(a.zzy);
-    wdot = wdot + W(a.yzy);
-    // 360-432
-    udot = udot + p.sck * W(a.uzz);
-    vdot = vdot + p.sck * W(a.vzz);
-    udot = udot + p.sck * W(a.uxx);
-    vdot = vdot + p.sck * W(a.vxx);
-    wdot = wdot + p.sck * W(a.wxx);
-    udot = udot + p.sck * W(a.uyy);
-    vdot = vdot + p.sck * W(a.vyy);
-    wdot = wdot + p.sck * W(a.wyy);
-    a.rdot[q] = rdot;
-    a.tdot[q] = tdot;
-    a.udot[q] = udot;
-    a.vdot[q] = vdot;
-    a.wdot[q] = wdot;
-}
-
-template <typename TW>
-__global__ void __launch_bounds__(NT) hydro_b_kernel(HydroArgs a)
-{
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
-    if (q >= a.g.n3) return;
-    const int k = (int)(q / a.g.nxy);
-    auto W = [&](const void* ptr) -> double { return (double)((const TW*)ptr)[q]; };
-    double udot = a.udot[q] + W(a.dxc1);
-    double vdot = a.vdot[q] + W(a.dyc2);
-    double wdot = a.wdot[q] + W(a.dzc3) + W(a.dzc4);
-    double rdot = a.rdot[q], tdot = a.tdot[q];
-    const double tt = a.t[q];
-    double rzu = W(a.rzu), rzc = W(a.rzc);
-    if (k == 0 || k == a.g.nz - 1) {
-        rzu = wdot / tt;
-        rzc = wdot / tt;
-    }
-    rdot = rdot - a.w[q] * rzu;
-    wdot = wdot - tt * rzc;
-    const double ri = 1.0 / (a.r[q]);
-    a.rdot[q] = rdot;
-    a.tdot[q] = tdot * ri;
-    a.udot[q] = udot * ri;
-    a.vdot[q] = vdot * ri;
-    a.wdot[q] = wdot * ri;
-}
-
-int launch_hydro_a(const HydroArgs& a, hipStream_t s)
-{
-    if (a.wf32) hipLaunchKernelGGL(hydro_a_kernel<float>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL(hydro_a_kernel<double>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
-}
-
-int launch_hydro_b(const HydroArgs& a, hipStream_t s)
-{
-    if (a.wf32) hipLaunchKernelGGL(hydro_b_kernel<float>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL(hydro_b_kernel<double>, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
-}
-
-// ---------------------------------------------------------------------------
 // magnetic part on the double-halfcomplex layout (src/magnetic.f90)
 // ---------------------------------------------------------------------------
 // spectral d/dx at (p,q): index swap along x, src/magnetic.f90:141-150

@@ -37,24 +37,37 @@ struct ZLaunch {
 };
 int launch_z(const ZLaunch& L, hipStream_t s);
 
-// ---- hydro right-hand side assembly (src/hydro_deriv.f90) -------------------
-struct HydroArgs {
+// ---- fused hydro right-hand side (src/hydro_deriv.f90:5-505) ----------------
+// One column-walking kernel computes every z-derivative of hydro_deriv
+// (d1bydz / d1upbydz / d2bydz with the reference's flags) from register
+// windows, the derivatives of the three composite fluxes (lines 181-184,
+// 258, 300, 391, 434) by linear splitting, the boundary d(rho)/dz
+// substitution (455-470) and the division by rho (487-497).  Inputs are
+// the state and the horizontal derivatives of the two spectral passes.
+//
+// Mixed precision: the state arrays are FP32 fluctuations about per-plane
+// FP64 backgrounds (t0, r0; B about its means bxm, bym, bzm) and the kernel
+// rebuilds full FP64 values on load; all arithmetic is FP64.
+struct HydroColArgs {
     Geom g;
     Phys p;
-    const double *u, *v, *w, *t, *r;
+    const void *u, *v, *w, *t, *r;      // state: FP64, or FP32 fluctuations (mixed)
+    const double *t0, *r0;              // mixed: backgrounds of t, r (nz entries)
     const void *bx, *by, *bz, *jsq;     // working precision
+    const double *bxm, *bym;            // mixed: Bx-bar(z), By-bar(z)
+    double bzm;                         // mixed: Bz-bar
     const double *kappa, *dkappa;
-    // x / y / z derivative planes (working precision)
-    const void *rx, *tx, *txx, *ux, *uxx, *vx, *vxx, *wx, *wxx, *xbb, *xzx;
-    const void *ry, *ty, *tyy, *uy, *uyy, *vy, *vyy, *wy, *wyy, *ybb, *yzy;
-    const void *tzu, *tzc, *tzz, *wzu, *wzc, *uzu, *uzz, *vzu, *vzz, *zzx, *zzy, *rzu, *rzc;
-    const void *dxc1, *dyc2, *dzc3, *dzc4;
-    double *rdot, *tdot, *udot, *vdot, *wdot;
-    void *c1, *c2, *c3, *c4;
-    int wf32;   // working arrays are float
+    // x pass: d/dx of r, t, u, v, w; d2/dx2 of t, v, w; uxxc = sck*uxx +
+    // scko3*d/dx(d/dx u); xbb = d/dx(F by bx); xzx = d/dx(F bz bx);
+    // dxm = d/dx(F/2 (bx^2 - by^2 - bz^2))
+    const void *rx, *tx, *txx, *ux, *uxxc, *vx, *vxx, *wx, *wxx, *xbb, *xzx, *dxm;
+    // y pass: the same along y with vyyc = sck*vyy + scko3*d/dy(d/dy v),
+    // dym = d/dy(F/2 (by^2 - bx^2 - bz^2)), uxy = d/dy(ux), vxy = d/dy(vx)
+    const void *ry, *ty, *tyy, *uy, *uyy, *vy, *vyyc, *wy, *wyy, *ybb, *yzy, *dym, *uxy, *vxy;
+    void *rdot, *tdot, *udot, *vdot, *wdot;   // newest history level (working precision)
+    int zc;                             // planes per thread
 };
-int launch_hydro_a(const HydroArgs& a, hipStream_t s);
-int launch_hydro_b(const HydroArgs& a, hipStream_t s);
+int launch_hydro_col(const HydroColArgs& a, int mixed, hipStream_t s);
 
 // ---- magnetic part (src/magnetic.f90) --------------------------------------
 struct MagArgs {

@@ -9,10 +9,18 @@
 namespace mhd {
 
 enum SpecMode { SM_DERIV1 = 0, SM_DERIV2 = 1, SM_R2HC = 2, SM_HC2R = 3 };
-enum MulKind { MUL_D1 = 0, MUL_D2 = 1, MUL_MASK = 2 };
+// MUL_D2C: mc[0] * d2 + mc[1] * d1(d1(.)); the two differ at the Nyquist
+// mode only, which d2 keeps (akx2) and d1 zeroes.
+enum MulKind { MUL_D1 = 0, MUL_D2 = 1, MUL_MASK = 2, MUL_D2C = 3 };
 // LD_BG: FP64 input minus a per-plane background (the mixed path's
 // fluctuation loads); the others read inputs of the compute type.
-enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3 };
+// Products take optional per-plane means m_i of their factors (the mixed
+// path stores B without its means): the load is the product of (in_i + m_i)
+// minus its per-plane constant, which no horizontal derivative sees.
+//   LD_PROD  scale*(in0+m0)*(in1+m1)
+//   LD_CROSS (in0+m0)*(in1+m1) - (in2+m2)*(in3+m3)
+//   LD_MAXW  scale*((in0+m0)^2 - (in1+m1)^2 - (in2+m2)^2)
+enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3, LD_MAXW = 4 };
 
 constexpr int SPEC_MAX_JOBS = 8;
 
@@ -21,8 +29,10 @@ constexpr int SPEC_MAX_JOBS = 8;
 struct SpecJob {
     const void* in[4];
     int in_f32;
-    int load_op;      // LoadOp: in0 | scale*in0*in1 | in0*in1 - in2*in3
+    int load_op;      // LoadOp
     double scale;
+    const double* mean[4]; // optional per-plane means of in[0..3] (nz entries)
+    double mc[2];     // MUL_D2C coefficients
     const double* bg; // optional per-plane value subtracted on load (nz entries)
     void* out[2];
     int out_f32;

@@ -37,6 +37,7 @@ struct Loader {
     const double* __restrict__ s64;
     TC scale;
     double bgk;
+    TC m0, m1, m2, m3;   // per-plane means of the factors (0 when absent)
     __device__ __forceinline__ Loader(const SpecJob& J, int k)
     {
         a = (const TC*)J.in[0];
@@ -46,13 +47,30 @@ struct Loader {
         s64 = (const double*)J.in[0];
         scale = (TC)J.scale;
         bgk = (LOP == LD_BG && J.bg) ? J.bg[k] : 0.0;
+        m0 = m1 = m2 = m3 = (TC)0;
+        if constexpr (LOP == LD_PROD || LOP == LD_CROSS || LOP == LD_MAXW) {
+            if (J.mean[0]) m0 = (TC)J.mean[0][k];
+            if (J.mean[1]) m1 = (TC)J.mean[1][k];
+            if (J.mean[2]) m2 = (TC)J.mean[2][k];
+            if (J.mean[3]) m3 = (TC)J.mean[3][k];
+        }
     }
     __device__ __forceinline__ TC operator()(long i) const
     {
         if constexpr (LOP == LD_PLAIN) return a[i];
         else if constexpr (LOP == LD_BG) return (TC)(s64[i] - bgk);
-        else if constexpr (LOP == LD_PROD) return (scale * a[i]) * b[i];
-        else return a[i] * b[i] - c[i] * d[i];
+        else if constexpr (LOP == LD_PROD) {
+            // (s*a)*b is the reference's product; the mean terms drop its constant
+            const TC x = a[i], y = b[i];
+            return (scale * x) * y + scale * (x * m1 + m0 * y);
+        } else if constexpr (LOP == LD_MAXW) {
+            const TC x = a[i], y = b[i], z = c[i];
+            return scale * x * x - scale * y * y - scale * z * z +
+                   (TC)2 * scale * (m0 * x - m1 * y - m2 * z);
+        } else {
+            const TC x = a[i], y = b[i], z = c[i], w = d[i];
+            return x * y - z * w + (x * m1 + m0 * y) - (z * m3 + m2 * w);
+        }
     }
 };
 
@@ -60,7 +78,8 @@ struct Loader {
 // 1/N.  The wavenumbers are those of src/fft.F90:196-221: mode m = min(k,N-k)
 // has akx1 = m*cx and akx2 = -(m*m*cx*cx), Nyquist included in akx2 only.
 template <typename TC, int N>
-__device__ __forceinline__ cpx<TC> apply_mul(int kind, cpx<TC> z, int k, double cx, int icut)
+__device__ __forceinline__ cpx<TC> apply_mul(int kind, cpx<TC> z, int k, double cx, int icut,
+                                              const double* mc)
 {
     constexpr double invN = 1.0 / N;
     const int m = (k < N - k) ? k : N - k;
@@ -70,8 +89,10 @@ __device__ __forceinline__ cpx<TC> apply_mul(int kind, cpx<TC> z, int k, double 
         // +i*a for k < N/2; the negative frequency N-k carries -i*a
         if (k < N / 2) return {-a * z.y, a * z.x};
         return {a * z.y, -a * z.x};
-    } else if (kind == MUL_D2) {
-        const TC a = (TC)((-((double)m * (double)m * cx * cx)) * invN);
+    } else if (kind == MUL_D2 || kind == MUL_D2C) {
+        double f = -((double)m * (double)m * cx * cx);
+        if (kind == MUL_D2C) f *= (k == N / 2) ? mc[0] : mc[0] + mc[1];
+        const TC a = (TC)(f * invN);
         return {a * z.x, a * z.y};
     } else {
         if (m < N / 2 - icut) return {z.x * (TC)invN, z.y * (TC)invN};
@@ -261,7 +282,7 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
                     if (j < NBL) {
 #pragma unroll
                         for (int r = 0; r < RL; ++r)
-                            v[r] = apply_mul<TC, N>(J.mul[o], o == 0 ? v[r] : z[r], j + r * NBL, cx, icut);
+                            v[r] = apply_mul<TC, N>(J.mul[o], o == 0 ? v[r] : z[r], j + r * NBL, cx, icut, J.mc);
                     }
                     F::inverse(v, j, lds, twl);
                     if (ok && j < NB0) {
@@ -325,6 +346,7 @@ static int launch_mode(const SpecLaunch& L, hipStream_t s)
     case SM_DERIV1 * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_PLAIN>(L, s);
     case SM_DERIV1 * 8 + LD_PROD: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_PROD>(L, s);
     case SM_DERIV1 * 8 + LD_BG: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_BG>(L, s);
+    case SM_DERIV1 * 8 + LD_MAXW: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_MAXW>(L, s);
     case SM_DERIV2 * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_DERIV2, LD_PLAIN>(L, s);
     case SM_DERIV2 * 8 + LD_BG: return launch_one<TC, PL, DIR, B, SM_DERIV2, LD_BG>(L, s);
     case SM_R2HC * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_R2HC, LD_PLAIN>(L, s);

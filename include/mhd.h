@@ -150,7 +150,7 @@ int mhd_op(mhd_ctx_t* ctx, int op, const double* in, double* out, double* out2, 
  * events recorded on the context's stream around every launch of it).
  * kernel: MHD_K_*; enable=1 resets and starts, enable=0 stops.  Query with
  * mhd_kernel_time(): total milliseconds and number of launches. */
-enum { MHD_K_NONE = 0, MHD_K_XDERIV2 = 1, MHD_K_YDERIV2 = 2, MHD_K_ZDERIV = 3, MHD_K_HYDRO_A = 4,
+enum { MHD_K_NONE = 0, MHD_K_XDERIV2 = 1, MHD_K_YDERIV2 = 2, MHD_K_ZDERIV = 3, MHD_K_HYDRO = 4,
        MHD_K_AB3 = 5 };
 int mhd_kernel_timing(mhd_ctx_t* ctx, int kernel, int enable);
 int mhd_kernel_time(mhd_ctx_t* ctx, double* total_ms, long* launches);
