@@ -8,6 +8,12 @@ step is one full pass of the time-loop body (hydro RHS, magnetic RHS, AB3
 update, dealias cadence, B and J reconstruction, boundary conditions).
 Inputs are resident in HBM before the timed region.
 
+The headline value is the FP64 plain method.  The same workload then runs in
+the mixed-precision mode (FP32 transforms, stencils and working arrays; FP64
+state, means, backgrounds and reductions); its rate, its speed-up and its
+relative error against the FP64 run (fields, Nu, kinetic and magnetic
+energy) are reported under "mixed".
+
 Prints one JSON line.  The roofline object covers the dominant kernel family
 (the x-direction spectral first+second derivative of T, u, v, w), timed with
 HIP events on the solver's stream over the timed steps.  cpu_baseline times
@@ -51,70 +57,120 @@ def cpu_baseline(params, nz=16, steps=6):
                       f"{steps} FP64 AB3 steps after 1 untimed, 1 thread, {dt:.1f} s"}
 
 
+def run(precision, steps, warmup, kernel):
+    """Static start, `warmup` untimed steps, `steps` timed steps."""
+    from fug30v0kxc_amd import MHD_PREC_FP64, MHD_PREC_MIXED, Solver
+    prec = MHD_PREC_FP64 if precision == "fp64" else MHD_PREC_MIXED
+    s = Solver(NX, NY, NZ, workload_params(), precision=prec)
+    s.static()
+    s.step(warmup, DT, NDU06)
+    s.sync()
+    s.kernel_timing(kernel, True)
+    s.sync()
+    t0 = time.perf_counter()
+    diag = s.step(steps, DT, NDU06)
+    s.sync()
+    t1 = time.perf_counter()
+    kms, klaunch = s.kernel_time()
+    s.kernel_timing(kernel, False)
+    st = s.get_state()
+    dev_bytes = s.device_bytes()
+    s.close()
+    return {"elapsed": t1 - t0, "diag": diag, "state": st, "kms": kms, "klaunch": klaunch,
+            "device_bytes": dev_bytes}
+
+
+def roofline_of(r, kernel, elem):
+    cells = NX * NY * NZ
+    # algorithmic bytes of one launch: 4 fields, 1 read + 2 writes each
+    bytes_per_launch = 4 * 3 * elem * cells
+    avg_ms = r["kms"] / max(r["klaunch"], 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"spec_kernel {kernel} (4 fields: d/dx and d2/dx2 of T,u,v,w)",
+            "avg_launch_ms": round(avg_ms, 4), "launches": r["klaunch"],
+            "bytes_per_launch": bytes_per_launch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--kernel", default="xderiv2", help="kernel family for the roofline object")
+    ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"),
+                    help="both: FP64 headline plus the mixed run; fp64 / mixed: that run alone "
+                         "(e.g. for a kernel profile of one mode)")
+    ap.add_argument("--kernel", default="xderiv2", choices=("xderiv2", "yderiv2"),
+                    help="kernel family for the roofline object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the whole box sits on one MI355X, see the sharding design notes)", file=sys.stderr)
         return 2
 
-    from fug30v0kxc_amd import Solver
-    p = workload_params()
-    s = Solver(NX, NY, NZ, p)
-    s.static()
-    s.step(args.warmup, DT, NDU06)
-    s.sync()
-
-    s.kernel_timing(args.kernel, True)
-    s.sync()
-    t0 = time.perf_counter()
-    diag = s.step(args.steps, DT, NDU06)
-    s.sync()
-    t1 = time.perf_counter()
-    kms, klaunch = s.kernel_time()
-    s.kernel_timing(args.kernel, False)
-
-    elapsed = t1 - t0
+    import numpy as np
     cells = NX * NY * NZ
-    value = cells * args.steps / elapsed / 1e6
-    st = s.get_state()
-    finite = all(float(abs(v).max()) < 1e30 for v in st.values())
+    if args.precision == "mixed":
+        rm = run("mixed", args.steps, args.warmup, args.kernel)
+        vm = cells * args.steps / rm["elapsed"] / 1e6
+        out = {"metric": METRIC, "value": round(vm, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": round(rm["elapsed"] / args.steps * 1e3, 4), "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 state and reductions)",
+               "data": "synthetic",
+               "config": {"workload": "256x256x160 compressible MHD, mixed precision, Ra=1e6, Q=1000, "
+                                      "T/Bz noise 1e-3, fixed dt 5e-5, diagnostics every 100 steps",
+                          "nx": NX, "ny": NY, "nz": NZ, "dt": DT, "ndu06": NDU06, "precision": "mixed"},
+               "roofline": roofline_of(rm, args.kernel, 4), "cpu_baseline": None,
+               "device_bytes": rm["device_bytes"]}
+        print(json.dumps(out))
+        return 0
+    r64 = run("fp64", args.steps, args.warmup, args.kernel)
+    value = cells * args.steps / r64["elapsed"] / 1e6
+    finite = all(float(np.abs(v).max()) < 1e30 for v in r64["state"].values())
+    d64 = r64["diag"]
 
-    # algorithmic bytes of one x-derivative launch: 4 fields, 1 read + 2 writes each
-    bytes_per_launch = {"xderiv2": 4 * 3 * 8 * cells, "yderiv2": 4 * 3 * 8 * cells}[args.kernel]
-    avg_ms = kms / max(klaunch, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"spec_kernel {args.kernel} (4 fields: d/dx and d2/dx2 of T,u,v,w)",
-                "avg_launch_ms": round(avg_ms, 4), "launches": klaunch,
-                "bytes_per_launch": bytes_per_launch}
+    mixed = None
+    if args.precision == "both":
+        rm = run("mixed", args.steps, args.warmup, args.kernel)
+        vm = cells * args.steps / rm["elapsed"] / 1e6
+        dm = rm["diag"]
+        # error of the mixed run against the FP64 run after the same steps
+        # (the FP64 path matches the FP64 oracle to 1e-10 in the parity tests)
+        fe = {n: float(np.abs(rm["state"][n] - r64["state"][n]).max() /
+                       max(np.abs(r64["state"][n]).max(), 1e-300)) for n in r64["state"]}
+        qe = {k: (float(abs(dm[k] - d64[k]) / abs(d64[k])) if d64.get(k) else None)
+              for k in ("nusselt", "kinenergy", "magenergy")}
+        mixed = {"value": round(vm, 2), "ms_per_step": round(rm["elapsed"] / args.steps * 1e3, 4),
+                 "speedup_vs_fp64": round(vm / value, 3),
+                 "rel_err_vs_fp64": {**{k: (float(f"{v:.3e}") if v is not None else None)
+                                        for k, v in qe.items()},
+                                     "fields": {k: float(f"{v:.3e}") for k, v in fe.items()}},
+                 "roofline": roofline_of(rm, args.kernel, 4),
+                 "device_bytes": rm["device_bytes"]}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(p)
+        cpu = cpu_baseline(workload_params())
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": round(elapsed / args.steps * 1e3, 4), "higher_is_better": True,
+           "ms_per_step": round(r64["elapsed"] / args.steps * 1e3, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "256x256x160 compressible MHD, FP64 plain, Ra=1e6, Q=1000, "
                                   "T/Bz noise 1e-3, fixed dt 5e-5, diagnostics every 100 steps",
                       "nx": NX, "ny": NY, "nz": NZ, "dt": DT, "ndu06": NDU06,
                       "precision": "fp64"},
-           "roofline": roofline, "cpu_baseline": cpu,
-           "diag": {"nusselt": diag.get("nusselt"), "kinenergy": diag.get("kinenergy"),
-                    "magenergy": diag.get("magenergy"), "finite": finite}}
+           "roofline": roofline_of(r64, args.kernel, 8), "cpu_baseline": cpu,
+           "mixed": mixed,
+           "device_bytes": r64["device_bytes"],
+           "diag": {"nusselt": d64.get("nusselt"), "kinenergy": d64.get("kinenergy"),
+                    "magenergy": d64.get("magenergy"), "finite": finite}}
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(out))
-    s.close()
     return 0
 
 

@@ -59,11 +59,15 @@ struct mhd_ctx {
     char* pool;
     size_t pool_bytes;
     double *akx1, *akx2, *aky1, *aky2, *kappa, *dkappa;
-    void *twx, *twy;
+    void *twx, *twy;                 // twiddles in the working precision
+    void *twx64, *twy64;             // FP64 twiddles (FP64 transforms of the mixed path)
     double *u, *v, *w, *t, *r, *pol, *tor, *bxbar, *bybar;
     void *bx, *by, *bz, *jsq;
-    double* hist[7][3];
+    void* hist[7][3];  // RHS history (FP32 in mixed precision)
+    int h32;
     double* mhist[2][3];
+    float *su, *sv, *sw, *st, *sr;   // mixed: FP32 fluctuation shadows of u v w t r
+    double *t0, *r0, *bzm;           // mixed: backgrounds of t, r; Bz-bar per plane
     int ring[3];
     void* wk[W_COUNT];
     double *red_partial, *red_result, *d_bzbar;
@@ -100,7 +104,7 @@ int hist_index(int id)
     return -1;
 }
 
-double* level_ptr(mhd_ctx* c, int f, int level) { return c->hist[f][c->ring[level - 1]]; }
+void* level_ptr(mhd_ctx* c, int f, int level) { return c->hist[f][c->ring[level - 1]]; }
 double* mlevel_ptr(mhd_ctx* c, int f, int level) { return c->mhist[f][c->ring[level - 1]]; }
 
 void set_constants(mhd_ctx* c)
@@ -201,7 +205,10 @@ double ran0(int* idum)
     return r;
 }
 
-SpecLaunch spec_base(mhd_ctx* c, int dir, int mode)
+int kerr(int e) { return e ? MHD_E_DEVICE : 0; }
+
+// f64: FP64 transform in the mixed path (launched with spec64)
+SpecLaunch spec_base(mhd_ctx* c, int dir, int mode, bool f64 = false)
 {
     SpecLaunch L;
     memset(&L, 0, sizeof(L));
@@ -209,7 +216,8 @@ SpecLaunch spec_base(mhd_ctx* c, int dir, int mode)
     L.dir = dir;
     L.mode = mode;
     L.cx = 8.0 * std::atan(1.0) / (dir == 0 ? c->p.xmax : c->p.ymax);   // src/fft.F90:184-185
-    L.tw = dir == 0 ? c->twx : c->twy;
+    if (f64) L.tw = dir == 0 ? c->twx64 : c->twy64;
+    else L.tw = dir == 0 ? c->twx : c->twy;
     L.icut = dir == 0 ? c->icutx : c->icuty;
     L.max_groups = c->max_groups;
     return L;
@@ -279,6 +287,21 @@ int spec(mhd_ctx* c, const SpecLaunch& L, int kernel = MHD_K_NONE)
     return e ? (e == -20 ? MHD_E_SIZE : MHD_E_DEVICE) : 0;
 }
 
+// FP64 transform whatever the working precision (L from spec_base(.., true))
+int spec64(mhd_ctx* c, const SpecLaunch& L)
+{
+    int e = launch_spectral(L, 0, c->s);
+    return e ? (e == -20 ? MHD_E_SIZE : MHD_E_DEVICE) : 0;
+}
+
+// refresh the FP32 shadows after the FP64 state changed outside the AB3 update
+int refresh_shadows(mhd_ctx* c)
+{
+    if (!c->su) return 0;
+    return kerr(launch_shadow(c->g, c->u, c->v, c->w, c->t, c->r, c->su, c->sv, c->sw, c->st, c->sr, c->t0, c->r0,
+                              c->s));
+}
+
 int zrun(mhd_ctx* c, const ZLaunch& L)
 {
     int e = timed(c, MHD_K_ZDERIV, [&] { return launch_z(L, c->s); });
@@ -291,7 +314,13 @@ HydroColArgs colargs(mhd_ctx* c)
     memset(&a, 0, sizeof(a));
     a.g = c->g;
     a.p = c->ph;
-    a.u = c->u; a.v = c->v; a.w = c->w; a.t = c->t; a.r = c->r;
+    if (c->su) {
+        a.u = c->su; a.v = c->sv; a.w = c->sw; a.t = c->st; a.r = c->sr;
+        a.t0 = c->t0; a.r0 = c->r0;
+        a.bxm = c->bxbar; a.bym = c->bybar; a.bzm = c->sc.bzbar;
+    } else {
+        a.u = c->u; a.v = c->v; a.w = c->w; a.t = c->t; a.r = c->r;
+    }
     a.bx = c->bx; a.by = c->by; a.bz = c->bz; a.jsq = c->jsq;
     a.kappa = c->kappa; a.dkappa = c->dkappa;
     void** W = c->wk;
@@ -331,11 +360,11 @@ MagArgs margs(mhd_ctx* c)
     a.polz = W[W_VXX];
     a.bxh = W[W_WX]; a.byh = W[W_WXX]; a.bzh = W[W_XBB]; a.jzh = W[W_XZX]; a.dybzh = W[W_RY]; a.dxbzh = W[W_TY];
     a.bx = c->bx; a.by = c->by; a.jsq = c->jsq;
+    if (c->su) { a.bxm = c->bxbar; a.bym = c->bybar; }
     a.wf32 = c->wf32;
     return a;
 }
 
-int kerr(int e) { return e ? MHD_E_DEVICE : 0; }
 
 #define TRY(x)                    \
     do {                          \
@@ -352,26 +381,44 @@ int run_hydro_deriv(mhd_ctx* c)
     void** W = c->wk;
     const Phys& P = c->ph;
     const double f = c->p.f;
+    // mixed precision: the transforms read the FP32 fluctuation shadows, and
+    // the products the per-plane means of B (B is stored without them)
+    const bool mx = c->su != nullptr;
+    const void* T_ = mx ? (const void*)c->st : c->t;
+    const void* U_ = mx ? (const void*)c->su : c->u;
+    const void* V_ = mx ? (const void*)c->sv : c->v;
+    const void* W_ = mx ? (const void*)c->sw : c->w;
+    const void* R_ = mx ? (const void*)c->sr : c->r;
+    const int sf = mx ? 1 : 0;
+    const double* mbx = mx ? c->bxbar : nullptr;
+    const double* mby = mx ? c->bybar : nullptr;
+    const double* mbz = mx ? c->bzm : nullptr;
+    auto means = [](SpecJob& j, const double* m0, const double* m1, const double* m2) {
+        j.mean[0] = m0; j.mean[1] = m1; j.mean[2] = m2;
+    };
     // x pass: d/dx, d2/dx2 of t, u, v, w (u's second output also carries the
     // scko3 d/dx d/dx u of the composite flux, lines 181, 258); d/dx of r and
     // of the Maxwell-stress products F by bx, F bz bx, F/2 (bx^2-by^2-bz^2)
     {
         SpecLaunch L = spec_base(c, 0, SM_DERIV2);
-        L.job[0] = sjob(c->t, 0, W[W_TX], W[W_TXX], MUL_D1, MUL_D2, wf);
-        L.job[1] = sjob(c->u, 0, W[W_UX], W[W_UXX], MUL_D1, MUL_D2C, wf);
+        L.job[0] = sjob(T_, sf, W[W_TX], W[W_TXX], MUL_D1, MUL_D2, wf);
+        L.job[1] = sjob(U_, sf, W[W_UX], W[W_UXX], MUL_D1, MUL_D2C, wf);
         L.job[1].mc[0] = P.sck; L.job[1].mc[1] = P.scko3;
-        L.job[2] = sjob(c->v, 0, W[W_VX], W[W_VXX], MUL_D1, MUL_D2, wf);
-        L.job[3] = sjob(c->w, 0, W[W_WX], W[W_WXX], MUL_D1, MUL_D2, wf);
+        L.job[2] = sjob(V_, sf, W[W_VX], W[W_VXX], MUL_D1, MUL_D2, wf);
+        L.job[3] = sjob(W_, sf, W[W_WX], W[W_WXX], MUL_D1, MUL_D2, wf);
         L.njobs = 4;
         TRY(spec(c, L, MHD_K_XDERIV2));
         SpecLaunch M = spec_base(c, 0, SM_DERIV1);
-        M.job[0] = sjob(c->r, 0, W[W_RX], nullptr, MUL_D1, 0, wf);
+        M.job[0] = sjob(R_, sf, W[W_RX], nullptr, MUL_D1, 0, wf);
         M.job[1] = sjob(c->by, wf, W[W_XBB], nullptr, MUL_D1, 0, wf);
         M.job[1].load_op = LD_PROD; M.job[1].in[1] = c->bx; M.job[1].scale = f;
+        means(M.job[1], mby, mbx, nullptr);
         M.job[2] = sjob(c->bz, wf, W[W_XZX], nullptr, MUL_D1, 0, wf);
         M.job[2].load_op = LD_PROD; M.job[2].in[1] = c->bx; M.job[2].scale = f;
+        means(M.job[2], mbz, mbx, nullptr);
         M.job[3] = sjob(c->bx, wf, W[W_DXM], nullptr, MUL_D1, 0, wf);
         M.job[3].load_op = LD_MAXW; M.job[3].in[1] = c->by; M.job[3].in[2] = c->bz; M.job[3].scale = P.fo2;
+        means(M.job[3], mbx, mby, mbz);
         M.njobs = 4;
         TRY(spec(c, M));
     }
@@ -379,23 +426,26 @@ int run_hydro_deriv(mhd_ctx* c)
     // plus the mixed derivatives d/dy(ux), d/dy(vx) of the composite fluxes
     {
         SpecLaunch L = spec_base(c, 1, SM_DERIV2);
-        L.job[0] = sjob(c->t, 0, W[W_TY], W[W_TYY], MUL_D1, MUL_D2, wf);
-        L.job[1] = sjob(c->u, 0, W[W_UY], W[W_UYY], MUL_D1, MUL_D2, wf);
-        L.job[2] = sjob(c->v, 0, W[W_VY], W[W_VYY], MUL_D1, MUL_D2C, wf);
+        L.job[0] = sjob(T_, sf, W[W_TY], W[W_TYY], MUL_D1, MUL_D2, wf);
+        L.job[1] = sjob(U_, sf, W[W_UY], W[W_UYY], MUL_D1, MUL_D2, wf);
+        L.job[2] = sjob(V_, sf, W[W_VY], W[W_VYY], MUL_D1, MUL_D2C, wf);
         L.job[2].mc[0] = P.sck; L.job[2].mc[1] = P.scko3;
-        L.job[3] = sjob(c->w, 0, W[W_WY], W[W_WYY], MUL_D1, MUL_D2, wf);
+        L.job[3] = sjob(W_, sf, W[W_WY], W[W_WYY], MUL_D1, MUL_D2, wf);
         L.njobs = 4;
         TRY(spec(c, L, MHD_K_YDERIV2));
         SpecLaunch M = spec_base(c, 1, SM_DERIV1);
-        M.job[0] = sjob(c->r, 0, W[W_RY], nullptr, MUL_D1, 0, wf);
+        M.job[0] = sjob(R_, sf, W[W_RY], nullptr, MUL_D1, 0, wf);
         M.job[1] = sjob(W[W_UX], wf, W[W_UXY], nullptr, MUL_D1, 0, wf);
         M.job[2] = sjob(W[W_VX], wf, W[W_VXY], nullptr, MUL_D1, 0, wf);
         M.job[3] = sjob(c->by, wf, W[W_YBB], nullptr, MUL_D1, 0, wf);
         M.job[3].load_op = LD_PROD; M.job[3].in[1] = c->bx; M.job[3].scale = f;
+        means(M.job[3], mby, mbx, nullptr);
         M.job[4] = sjob(c->bz, wf, W[W_YZY], nullptr, MUL_D1, 0, wf);
         M.job[4].load_op = LD_PROD; M.job[4].in[1] = c->by; M.job[4].scale = f;
+        means(M.job[4], mbz, mby, nullptr);
         M.job[5] = sjob(c->by, wf, W[W_DYM], nullptr, MUL_D1, 0, wf);
         M.job[5].load_op = LD_MAXW; M.job[5].in[1] = c->bx; M.job[5].in[2] = c->bz; M.job[5].scale = P.fo2;
+        means(M.job[5], mby, mbx, mbz);
         M.njobs = 6;
         TRY(spec(c, M));
     }
@@ -413,14 +463,24 @@ int run_mag_deriv(mhd_ctx* c)
     {
         SpecLaunch L = spec_base(c, 0, SM_R2HC);
         // E = u x B (308-316), fused into the x pass of forwardfft
-        L.job[0] = sjob(c->v, wf, m.exh, nullptr, 0, 0, wf);
-        L.job[0].load_op = LD_CROSS; L.job[0].in[1] = c->bz; L.job[0].in[2] = c->w; L.job[0].in[3] = c->by;
-        L.job[1] = sjob(c->w, wf, m.eyh, nullptr, 0, 0, wf);
-        L.job[1].load_op = LD_CROSS; L.job[1].in[1] = c->bx; L.job[1].in[2] = c->u; L.job[1].in[3] = c->bz;
-        L.job[2] = sjob(c->u, wf, m.ezh, nullptr, 0, 0, wf);
-        L.job[2].load_op = LD_CROSS; L.job[2].in[1] = c->by; L.job[2].in[2] = c->v; L.job[2].in[3] = c->bx;
+        // mixed precision: FP32 velocity shadows, B without its means (added by the loads)
+        const bool mx = c->su != nullptr;
+        const void* U_ = mx ? (const void*)c->su : c->u;
+        const void* V_ = mx ? (const void*)c->sv : c->v;
+        const void* W_ = mx ? (const void*)c->sw : c->w;
+        const double* mbx = mx ? c->bxbar : nullptr;
+        const double* mby = mx ? c->bybar : nullptr;
+        const double* mbz = mx ? c->bzm : nullptr;
+        L.job[0] = sjob(V_, wf, m.exh, nullptr, 0, 0, wf);
+        L.job[0].load_op = LD_CROSS; L.job[0].in[1] = c->bz; L.job[0].in[2] = W_; L.job[0].in[3] = c->by;
+        L.job[0].mean[1] = mbz; L.job[0].mean[3] = mby;
+        L.job[1] = sjob(W_, wf, m.eyh, nullptr, 0, 0, wf);
+        L.job[1].load_op = LD_CROSS; L.job[1].in[1] = c->bx; L.job[1].in[2] = U_; L.job[1].in[3] = c->bz;
+        L.job[1].mean[1] = mbx; L.job[1].mean[3] = mbz;
+        L.job[2] = sjob(U_, wf, m.ezh, nullptr, 0, 0, wf);
+        L.job[2].load_op = LD_CROSS; L.job[2].in[1] = c->by; L.job[2].in[2] = V_; L.job[2].in[3] = c->bx;
+        L.job[2].mean[1] = mby; L.job[2].mean[3] = mbx;
         L.njobs = 3;
-        if (wf) return MHD_E_STATE;  // mixed path wires FP32 velocity shadows here
         TRY(spec(c, L));
         SpecLaunch M = spec_base(c, 1, SM_R2HC);
         M.job[0] = sjob(m.exh, wf, m.exh, nullptr, 0, 0, wf);
@@ -462,12 +522,14 @@ int inverse_b_and_j(mhd_ctx* c, MagArgs& m, bool with_b)
     n = 0;
     if (with_b) {
         // 575-593: inverse transforms, then the mean parts of B
+        // (the mixed path keeps B without its means: no additions)
+        const bool mx = c->su != nullptr;
         M.job[n] = sjob(m.bxh, wf, c->bx, nullptr, 0, 0, wf);
-        M.job[n++].addk = c->bxbar;
+        M.job[n++].addk = mx ? nullptr : c->bxbar;
         M.job[n] = sjob(m.byh, wf, c->by, nullptr, 0, 0, wf);
-        M.job[n++].addk = c->bybar;
+        M.job[n++].addk = mx ? nullptr : c->bybar;
         M.job[n] = sjob(m.bzh, wf, c->bz, nullptr, 0, 0, wf);
-        M.job[n++].adds = c->sc.bzbar;
+        M.job[n++].adds = mx ? 0.0 : c->sc.bzbar;
     }
     M.job[n++] = sjob(m.dybzh, wf, m.dybzh, nullptr, 0, 0, wf);
     M.job[n++] = sjob(m.dxbzh, wf, m.dxbzh, nullptr, 0, 0, wf);
@@ -497,16 +559,35 @@ int run_mag_boundary(mhd_ctx* c, int hydro, int mag)
     b.theta = c->p.theta;
     b.u = c->u; b.v = c->v; b.w = c->w; b.t = c->t;
     b.pol = c->pol; b.tor = c->tor; b.bxbar = c->bxbar; b.bybar = c->bybar;
+    b.su = c->su; b.sv = c->sv; b.sw = c->sw; b.st = c->st; b.t0 = c->t0;
     b.hydro = hydro;
     b.mag = mag;
     return kerr(launch_boundary(b, c->s));
 }
 
-// mag_init, src/magnetic.f90:5-279
+// Bz-bar as a per-plane vector (the mixed path's loads take per-plane means)
+int fill_bzm(mhd_ctx* c)
+{
+    if (!c->bzm) return 0;
+    return kerr(launch_fill(c->bzm, 0, c->sc.bzbar, c->nz, c->s));
+}
+
+// mag_init, src/magnetic.f90:5-279.  The mixed path stores B without its
+// means, which live in bxbar, bybar and bzbar and are kept as they are.
 int run_mag_init(mhd_ctx* c)
 {
     const int wf = c->wf32;
     MagArgs m = margs(c);
+    const bool mx = c->su != nullptr;
+    std::vector<double> keep;
+    const double keep_bz = c->sc.bzbar;
+    if (mx) {
+        keep.resize(2 * c->nz);
+        MHD_HIP_CHECK(hipMemcpyAsync(keep.data(), c->bxbar, c->nz * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        MHD_HIP_CHECK(hipMemcpyAsync(keep.data() + c->nz, c->bybar, c->nz * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->s));
+        MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    }
     SpecLaunch L = spec_base(c, 0, SM_R2HC);
     L.job[0] = sjob(c->bx, wf, m.exh, nullptr, 0, 0, wf);
     L.job[1] = sjob(c->by, wf, m.eyh, nullptr, 0, 0, wf);
@@ -522,6 +603,14 @@ int run_mag_init(mhd_ctx* c)
     TRY(kerr(launch_mag_init_spec(m, c->s, c->d_bzbar)));
     MHD_HIP_CHECK(hipMemcpyAsync(&c->sc.bzbar, c->d_bzbar, sizeof(double), hipMemcpyDeviceToHost, c->s));
     MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    if (mx) {
+        MHD_HIP_CHECK(hipMemcpyAsync(c->bxbar, keep.data(), c->nz * sizeof(double), hipMemcpyHostToDevice, c->s));
+        MHD_HIP_CHECK(hipMemcpyAsync(c->bybar, keep.data() + c->nz, c->nz * sizeof(double), hipMemcpyHostToDevice,
+                                     c->s));
+        c->sc.bzbar = keep_bz;
+        TRY(fill_bzm(c));
+        MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    }
     TRY(inverse_b_and_j(c, m, false));
     return run_mag_boundary(c, 0, 1);
 }
@@ -529,18 +618,15 @@ int run_mag_init(mhd_ctx* c)
 // 5-field dealias (src/main.F90:495-502), in place
 int run_dealias5(mhd_ctx* c)
 {
+    // the FP64 state is filtered in FP64 in both precision modes
     double* f5[5] = {c->r, c->t, c->u, c->v, c->w};
     for (int dir = 0; dir < 2; ++dir) {
-        SpecLaunch L = spec_base(c, dir, SM_DERIV1);
+        SpecLaunch L = spec_base(c, dir, SM_DERIV1, true);
         for (int i = 0; i < 5; ++i) L.job[i] = sjob(f5[i], 0, f5[i], nullptr, MUL_MASK, 0, 0);
         L.njobs = 5;
-        if (c->wf32) {
-            // the transform works in the compute type; the state stays FP64
-            for (int i = 0; i < 5; ++i) L.job[i].out_f32 = 0;
-        }
-        TRY(spec(c, L));
+        TRY(spec64(c, L));
     }
-    return 0;
+    return refresh_shadows(c);
 }
 
 RedArgs rargs(mhd_ctx* c, int op)
@@ -553,6 +639,7 @@ RedArgs rargs(mhd_ctx* c, int op)
     a.u = c->u; a.v = c->v; a.w = c->w; a.r = c->r; a.t = c->t;
     a.bx = c->bx; a.by = c->by; a.bz = c->bz;
     a.wf32 = c->wf32;
+    if (c->su) { a.bxm = c->bxbar; a.bym = c->bybar; a.bzm = c->sc.bzbar; }
     a.partial = c->red_partial;
     a.result = c->red_result;
     return a;
@@ -571,6 +658,7 @@ int run_global_averages(mhd_ctx* c, double out[4])
     RedArgs a = rargs(c, RED_GLOBALAVG);
     TRY(kerr(launch_reduce(a, c->s)));
     TRY(kerr(launch_com_shift(c->g, c->u, c->v, c->red_result, c->s)));
+    TRY(refresh_shadows(c));
     return out ? fetch_result(c, out, 4) : 0;
 }
 
@@ -653,6 +741,12 @@ int run_ab3(mhd_ctx* c, const double cf[3])
     a.mvar[1] = c->bybar;
     for (int f = 0; f < 2; ++f)
         for (int l = 0; l < 3; ++l) a.mhist[f][l] = mlevel_ptr(c, f, l + 1);
+    a.h32 = c->h32;
+    if (c->su) {
+        a.shadow[0] = c->su; a.shadow[1] = c->sv; a.shadow[2] = c->sw; a.shadow[3] = c->sr; a.shadow[4] = c->st;
+        a.bg_t = c->t0;
+        a.bg_r = c->r0;
+    }
     a.mag = c->ph.mag;
     a.nfields = a.mag ? 7 : 5;
     a.icutx = c->icutx;
@@ -709,7 +803,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     if (2 * (ny / 2) != ny) return MHD_E_ODD_NY;
     if (nz < 6) return MHD_E_NZ;
     if (!spectral_len_supported(nx) || !spectral_len_supported(ny)) return MHD_E_SIZE;
-    if (precision != MHD_PREC_FP64) return MHD_E_ARG;
+    if (precision != MHD_PREC_FP64 && precision != MHD_PREC_MIXED) return MHD_E_ARG;
     int ndev = 0, dev = 0, ncu = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MHD_E_DEVICE;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -722,7 +816,8 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->nxy = nx * ny;
     c->n3 = (long)nx * ny * nz;
     c->prec = precision;
-    c->wf32 = 0;
+    c->wf32 = precision == MHD_PREC_MIXED;
+    c->h32 = c->wf32;
     c->wsz = c->wf32 ? sizeof(float) : sizeof(double);
     c->p = *p;
     c->g.nx = nx; c->g.ny = ny; c->g.nz = nz; c->g.nxy = nx * ny; c->g.n3 = c->n3;
@@ -747,6 +842,8 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     // pool layout
     const size_t f64 = al256(c->n3 * sizeof(double));
     const size_t fw = al256(c->n3 * c->wsz);
+    const size_t fh = al256(c->n3 * (c->h32 ? sizeof(float) : sizeof(double)));
+    const size_t fs = c->wf32 ? al256(c->n3 * sizeof(float)) : 0;   // shadows
     const size_t vz = al256(nz * sizeof(double));
     size_t bytes = 0;
     bytes += 2 * al256(nx * sizeof(double)) + 2 * al256(ny * sizeof(double)) + 2 * vz;   // tables
@@ -754,7 +851,8 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     bytes += 7 * f64 + 2 * vz;                       // state
     bytes += f64;                                    // FP64 staging for host transfers
     bytes += 4 * fw;                                  // bx by bz jsq
-    bytes += 21 * f64 + 6 * vz;                       // history
+    bytes += 21 * fh + 6 * vz;                        // history
+    bytes += 5 * fs + 3 * vz;                         // shadows, backgrounds
     bytes += W_COUNT * fw;                            // work
     bytes += al256(nz * 8 * sizeof(double)) + 2 * al256(8 * sizeof(double));  // reductions
     if (hipMalloc((void**)&c->pool, bytes) != hipSuccess) {
@@ -782,19 +880,28 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     void* twy32 = take(al256(2 * ny * sizeof(double)));
     c->twx = c->wf32 ? twx32 : twx64;
     c->twy = c->wf32 ? twy32 : twy64;
+    c->twx64 = twx64;
+    c->twy64 = twy64;
     c->u = (double*)take(f64); c->v = (double*)take(f64); c->w = (double*)take(f64);
     c->t = (double*)take(f64); c->r = (double*)take(f64);
     c->pol = (double*)take(f64); c->tor = (double*)take(f64);
     c->bxbar = (double*)take(vz); c->bybar = (double*)take(vz);
     c->bx = take(fw); c->by = take(fw); c->bz = take(fw); c->jsq = take(fw);
     for (int f = 0; f < 7; ++f)
-        for (int l = 0; l < 3; ++l) c->hist[f][l] = (double*)take(f64);
+        for (int l = 0; l < 3; ++l) c->hist[f][l] = take(fh);
     for (int f = 0; f < 2; ++f)
         for (int l = 0; l < 3; ++l) c->mhist[f][l] = (double*)take(vz);
     for (int i = 0; i < W_COUNT; ++i) c->wk[i] = take(fw);
     c->red_partial = (double*)take(al256(nz * 8 * sizeof(double)));
     c->red_result = (double*)take(al256(8 * sizeof(double)));
     c->d_bzbar = (double*)take(al256(8 * sizeof(double)));
+    c->t0 = (double*)take(vz);
+    c->r0 = (double*)take(vz);
+    c->bzm = (double*)take(vz);
+    if (c->wf32) {
+        c->su = (float*)take(fs); c->sv = (float*)take(fs); c->sw = (float*)take(fs);
+        c->st = (float*)take(fs); c->sr = (float*)take(fs);
+    }
     if ((size_t)(cur - c->pool) > bytes) {
         fprintf(stderr, "mhd: pool overflow\n");
         mhd_destroy(c);
@@ -818,6 +925,17 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     e |= hipMemcpy(c->aky2, b2.data(), ny * sizeof(double), hipMemcpyHostToDevice);
     e |= hipMemcpy(c->kappa, ka.data(), nz * sizeof(double), hipMemcpyHostToDevice);
     e |= hipMemcpy(c->dkappa, dka.data(), nz * sizeof(double), hipMemcpyHostToDevice);
+    {
+        // backgrounds of the mixed path's fluctuations: the polytrope of
+        // src/initial.F90:47-53, T0 = 1 + theta z, rho0 = T0^m
+        std::vector<double> t0(nz), r0(nz);
+        for (int k = 0; k < nz; ++k) {
+            t0[k] = 1.0 + (double)k * c->dz * p->theta;
+            r0[k] = std::pow(t0[k], p->cm);
+        }
+        e |= hipMemcpy(c->t0, t0.data(), nz * sizeof(double), hipMemcpyHostToDevice);
+        e |= hipMemcpy(c->r0, r0.data(), nz * sizeof(double), hipMemcpyHostToDevice);
+    }
     twiddles(nx, tw64, tw32);
     e |= hipMemcpy(twx64, tw64.data(), 2 * nx * sizeof(double), hipMemcpyHostToDevice);
     e |= hipMemcpy(twx32, tw32.data(), 2 * nx * sizeof(float), hipMemcpyHostToDevice);
@@ -877,6 +995,34 @@ static int download(mhd_ctx* c, double* dst, const void* src, int src_f32, long 
     return 0;
 }
 
+// mixed precision: the per-plane mean of B component comp (0 bx, 1 by, 2 bz)
+static int b_means(mhd_ctx* c, int comp, std::vector<double>& m)
+{
+    m.assign(c->nz, 0.0);
+    if (comp == 2) {
+        for (int k = 0; k < c->nz; ++k) m[k] = c->sc.bzbar;
+        return 0;
+    }
+    MHD_HIP_CHECK(hipMemcpyAsync(m.data(), comp == 0 ? c->bxbar : c->bybar, c->nz * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->s));
+    MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+// B component to the host, with its means added back in the mixed path
+static int download_b(mhd_ctx* c, double* dst, int comp)
+{
+    if (!dst) return 0;
+    const void* src = comp == 0 ? c->bx : (comp == 1 ? c->by : c->bz);
+    TRY(download(c, dst, src, c->wf32, c->n3));
+    if (!c->su) return 0;
+    std::vector<double> m;
+    TRY(b_means(c, comp, m));
+    for (int k = 0; k < c->nz; ++k)
+        for (long q = 0; q < c->nxy; ++q) dst[(long)k * c->nxy + q] += m[k];
+    return 0;
+}
+
 int mhd_static(mhd_ctx_t* c)
 {
     TRY(check_ctx(c));
@@ -914,10 +1060,41 @@ int mhd_set_state(mhd_ctx_t* c, const double* r, const double* t, const double* 
     TRY(upload(c, c->u, 0, u, n));
     TRY(upload(c, c->v, 0, v, n));
     TRY(upload(c, c->w, 0, w, n));
-    TRY(upload(c, c->bx, c->wf32, bx, n));
-    TRY(upload(c, c->by, c->wf32, by, n));
-    TRY(upload(c, c->bz, c->wf32, bz, n));
+    if (!c->su) {
+        TRY(upload(c, c->bx, c->wf32, bx, n));
+        TRY(upload(c, c->by, c->wf32, by, n));
+        TRY(upload(c, c->bz, c->wf32, bz, n));
+    } else {
+        // mixed: B is stored as FP32 fluctuations about its means, which are
+        // the plane averages of bx, by and the bottom-plane average of bz
+        // (the DC terms mag_init takes, src/magnetic.f90:63-90)
+        const int nz = c->nz;
+        const long m = c->nxy;
+        std::vector<double> mxv(nz, 0.0), myv(nz, 0.0), fl(n);
+        double mz = 0.0;
+        for (int k = 0; k < nz; ++k) {
+            for (long q = 0; q < m; ++q) {
+                mxv[k] += bx[(long)k * m + q];
+                myv[k] += by[(long)k * m + q];
+            }
+            mxv[k] /= (double)m;
+            myv[k] /= (double)m;
+        }
+        for (long q = 0; q < m; ++q) mz += bz[q];
+        mz /= (double)m;
+        for (long q = 0; q < n; ++q) fl[q] = bx[q] - mxv[q / m];
+        TRY(upload(c, c->bx, 1, fl.data(), n));
+        for (long q = 0; q < n; ++q) fl[q] = by[q] - myv[q / m];
+        TRY(upload(c, c->by, 1, fl.data(), n));
+        for (long q = 0; q < n; ++q) fl[q] = bz[q] - mz;
+        TRY(upload(c, c->bz, 1, fl.data(), n));
+        MHD_HIP_CHECK(hipMemcpy(c->bxbar, mxv.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+        MHD_HIP_CHECK(hipMemcpy(c->bybar, myv.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+        c->sc.bzbar = mz;
+        TRY(fill_bzm(c));
+    }
     if (c->ph.mag) TRY(run_mag_init(c));
+    TRY(refresh_shadows(c));
     c->have_state = 1;
     return mhd_sync(c);
 }
@@ -932,9 +1109,9 @@ int mhd_get_state(mhd_ctx_t* c, double* r, double* t, double* u, double* v, doub
     TRY(download(c, u, c->u, 0, n));
     TRY(download(c, v, c->v, 0, n));
     TRY(download(c, w, c->w, 0, n));
-    TRY(download(c, bx, c->bx, c->wf32, n));
-    TRY(download(c, by, c->by, c->wf32, n));
-    TRY(download(c, bz, c->bz, c->wf32, n));
+    TRY(download_b(c, bx, 0));
+    TRY(download_b(c, by, 1));
+    TRY(download_b(c, bz, 2));
     return 0;
 }
 
@@ -968,6 +1145,7 @@ static int field_ref(mhd_ctx* c, int id, int level, void** ptr, int* f32, long* 
     const int h = hist_index(id);
     if (h < 0) return MHD_E_ARG;
     *ptr = level_ptr(c, h, level);
+    *f32 = c->h32;
     return 0;
 }
 
@@ -979,6 +1157,7 @@ int mhd_get_field(mhd_ctx_t* c, int id, int level, double* dst)
     int f32;
     long n;
     TRY(field_ref(c, id, level, &p, &f32, &n));
+    if (id == MHD_F_BX || id == MHD_F_BY || id == MHD_F_BZ) return download_b(c, dst, id - MHD_F_BX);
     return download(c, dst, p, f32, n);
 }
 
@@ -990,7 +1169,12 @@ int mhd_set_field(mhd_ctx_t* c, int id, int level, const double* src)
     int f32;
     long n;
     TRY(field_ref(c, id, level, &p, &f32, &n));
-    return upload(c, p, f32, src, n);
+    // mixed: B is kept as fluctuations about its means; set it with mhd_set_state
+    if (c->su && (id == MHD_F_BX || id == MHD_F_BY || id == MHD_F_BZ)) return MHD_E_ARG;
+    TRY(upload(c, p, f32, src, n));
+    if (id == MHD_F_U || id == MHD_F_V || id == MHD_F_W || id == MHD_F_T || id == MHD_F_R)
+        TRY(refresh_shadows(c));
+    return mhd_sync(c);
 }
 
 int mhd_get_scalars(mhd_ctx_t* c, mhd_scalars_t* s)
@@ -1006,7 +1190,8 @@ int mhd_set_scalars(mhd_ctx_t* c, const mhd_scalars_t* s)
     TRY(check_ctx(c));
     if (!s) return MHD_E_ARG;
     c->sc = *s;
-    return 0;
+    TRY(fill_bzm(c));
+    return mhd_sync(c);
 }
 
 int mhd_hydro_deriv(mhd_ctx_t* c)
@@ -1133,6 +1318,9 @@ int mhd_sminmax(mhd_ctx_t* c, int id, double out[2])
     RedArgs a = rargs(c, RED_MINMAX);
     a.f0 = p;
     a.f_f32 = f32;
+    if (c->su && id == MHD_F_BX) a.f0m = c->bxbar;
+    if (c->su && id == MHD_F_BY) a.f0m = c->bybar;
+    if (c->su && id == MHD_F_BZ) a.f0s = c->sc.bzbar;
     TRY(kerr(launch_reduce(a, c->s)));
     return fetch_result(c, out, 2);
 }

@@ -235,8 +235,8 @@ __global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs a)
     const double pzz = dz2<double>(k, nz, 3, a.p.sc1, [&](int kk) { return pol[col + (long)kk * nxy]; });
     const double tzz = dz2<double>(k, nz, 3, a.p.sc1, [&](int kk) { return tor[col + (long)kk * nxy]; });
     const double zck = a.p.zck;
-    a.poldot[q] = zck * pol[q] * k2 + zck * pzz + il_c;
-    a.tordot[q] = zck * tor[q] * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q];
+    ((TW*)a.poldot)[q] = (TW)(zck * pol[q] * k2 + zck * pzz + il_c);
+    ((TW*)a.tordot)[q] = (TW)(zck * tor[q] * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q]);
 }
 
 // spectral B and J pieces of bandj, src/magnetic.f90:519-617
@@ -289,8 +289,13 @@ __global__ void __launch_bounds__(NT) jsq_kernel(MagArgs a)
     const long col = q - (long)k * nxy;
     const TW* by = (const TW*)a.by;
     const TW* bx = (const TW*)a.bx;
-    const double wk1 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)by[col + (long)kk * nxy]; });
-    const double wk2 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)bx[col + (long)kk * nxy]; });
+    double wk1 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)by[col + (long)kk * nxy]; });
+    double wk2 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)bx[col + (long)kk * nxy]; });
+    if (a.bym) {
+        // mixed: bx, by hold no means; d/dz of the means (flag 5) is added back
+        wk1 += dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return a.bym[kk]; });
+        wk2 += dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return a.bxm[kk]; });
+    }
     const double ex = (double)((const TW*)a.dybzh)[q];
     const double ey = (double)((const TW*)a.dxbzh)[q];
     const double ez = (double)((const TW*)a.jzh)[q];
@@ -401,40 +406,50 @@ int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out)
 // ---------------------------------------------------------------------------
 // AB3 update, src/main.F90:413-477, plus magdealias (src/magnetic.f90:666-717)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NT) ab3_kernel(AB3Args a)
+template <typename TH>
+__global__ void __launch_bounds__(NT) ab3_kernel(AB3Args av)
 {
+    (void)av;
+    const AB3Args& a = kernarg<AB3Args>();
     const long q = (long)blockIdx.x * NT + threadIdx.x;
     if (q < a.g.n3) {
         const double c0 = a.c0, c1 = a.c1, c2 = a.c2;
+        double xn[7];
 #pragma unroll
         for (int f = 0; f < 7; ++f) {
             if (f >= a.nfields) break;
             double* x = a.var[f];
-            x[q] = x[q] + c0 * a.hist[f][2][q] + c1 * a.hist[f][1][q] + c2 * a.hist[f][0][q];
+            xn[f] = x[q] + c0 * (double)((const TH*)a.hist[f][2])[q] + c1 * (double)((const TH*)a.hist[f][1])[q] +
+                    c2 * (double)((const TH*)a.hist[f][0])[q];
         }
-        double* r = a.var[3];
-        if (r[q] < 0.05) {
-            r[q] = 0.05;
-            a.hist[3][0][q] = 0.0;
-            a.hist[3][1][q] = 0.0;
-            a.hist[3][2][q] = 0.0;
+        // src/main.F90:417-425: density floor, its history zeroed
+        if (xn[3] < 0.05) {
+            xn[3] = 0.05;
+            ((TH*)a.hist[3][0])[q] = (TH)0;
+            ((TH*)a.hist[3][1])[q] = (TH)0;
+            ((TH*)a.hist[3][2])[q] = (TH)0;
         }
         if (a.mag) {
             const int nx = a.g.nx, ny = a.g.ny;
             const int rem = (int)(q % a.g.nxy);
             const int jq = rem / nx, ip = rem - jq * nx;
             if (in_band(ip, jq, nx, ny, a.icutx, a.icuty)) {
-                a.var[5][q] = 0.0;
-                a.var[6][q] = 0.0;
+                xn[5] = 0.0;
+                xn[6] = 0.0;
             }
+        }
+#pragma unroll
+        for (int f = 0; f < 7; ++f) {
+            if (f >= a.nfields) break;
+            a.var[f][q] = xn[f];
         }
         if (a.shadow[0]) {
             const int k = (int)(q / a.g.nxy);
-            ((float*)a.shadow[0])[q] = (float)a.var[0][q];
-            ((float*)a.shadow[1])[q] = (float)a.var[1][q];
-            ((float*)a.shadow[2])[q] = (float)a.var[2][q];
-            ((float*)a.shadow[3])[q] = (float)(a.var[3][q] - (a.bg_r ? a.bg_r[k] : 0.0));
-            ((float*)a.shadow[4])[q] = (float)(a.var[4][q] - (a.bg_t ? a.bg_t[k] : 0.0));
+            ((float*)a.shadow[0])[q] = (float)xn[0];
+            ((float*)a.shadow[1])[q] = (float)xn[1];
+            ((float*)a.shadow[2])[q] = (float)xn[2];
+            ((float*)a.shadow[3])[q] = (float)(xn[3] - a.bg_r[k]);
+            ((float*)a.shadow[4])[q] = (float)(xn[4] - a.bg_t[k]);
         }
     }
     if (q < a.g.nz && a.mag) {
@@ -448,7 +463,8 @@ __global__ void __launch_bounds__(NT) ab3_kernel(AB3Args a)
 int launch_ab3(const AB3Args& a, hipStream_t s)
 {
     long n = a.g.n3 > a.g.nz ? a.g.n3 : a.g.nz;
-    hipLaunchKernelGGL(ab3_kernel, dim3(nblk(n)), dim3(NT), 0, s, a);
+    if (a.h32) hipLaunchKernelGGL(ab3_kernel<float>, dim3(nblk(n)), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL(ab3_kernel<double>, dim3(nblk(n)), dim3(NT), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -22;
 }
 
@@ -472,6 +488,16 @@ __global__ void __launch_bounds__(NT) boundary_kernel(BCArgs a)
         a.u[L + p] = elf * (18.0 * a.u[L - m + p] - 9.0 * a.u[L - 2 * m + p] + 2.0 * a.u[L - 3 * m + p]);
         a.v[L + p] = elf * (18.0 * a.v[L - m + p] - 9.0 * a.v[L - 2 * m + p] + 2.0 * a.v[L - 3 * m + p]);
         a.t[L + p] = 1.0 + a.theta;
+        if (a.su) {
+            const long rows[2] = {p, L + p};
+            const int ks[2] = {0, a.g.nz - 1};
+            for (int i = 0; i < 2; ++i) {
+                a.su[rows[i]] = (float)a.u[rows[i]];
+                a.sv[rows[i]] = (float)a.v[rows[i]];
+                a.sw[rows[i]] = (float)a.w[rows[i]];
+                a.st[rows[i]] = (float)(a.t[rows[i]] - a.t0[ks[i]]);
+            }
+        }
     }
     if (p < m && a.mag) {
         const long L = (long)(a.g.nz - 1) * m;
@@ -558,7 +584,12 @@ __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
         for (int i = tid; i < m; i += NT) {
             const long q = base + i;
             const double uu = a.u[q], vv = a.v[q], rr = a.r[q], ww = a.w[q];
-            const double bx = rdw<TW>(a.bx, q), by = rdw<TW>(a.by, q), bz = rdw<TW>(a.bz, q);
+            double bx = rdw<TW>(a.bx, q), by = rdw<TW>(a.by, q), bz = rdw<TW>(a.bz, q);
+            if (a.bxm) {
+                bx += a.bxm[k];
+                by += a.bym[k];
+                bz += a.bzm;
+            }
             v[0] += rr;
             v[1] += rr * uu;
             v[2] += rr * vv;
@@ -574,7 +605,9 @@ __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
         for (int i = tid; i < m; i += NT) {
             const long q = base + i;
             double x;
-            if (a.op == RED_MINMAX) x = a.f_f32 ? (double)((const float*)a.f0)[q] : ((const double*)a.f0)[q];
+            if (a.op == RED_MINMAX)
+                x = (a.f_f32 ? (double)((const float*)a.f0)[q] : ((const double*)a.f0)[q]) +
+                    (a.f0m ? a.f0m[k] : 0.0) + a.f0s;
             else
                 x = (a.f_f32 ? (double)((const float*)a.f0)[q] : ((const double*)a.f0)[q]) +
                     (a.f_f32 ? (double)((const float*)a.f1)[q] : ((const double*)a.f1)[q]) +
@@ -595,7 +628,12 @@ __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
         for (int i = tid; i < m; i += NT) {
             const long q = base + i;
             const double u0 = a.u[q], v0 = a.v[q], w0 = a.w[q], r0 = a.r[q], t0 = a.t[q];
-            const double bz0 = rdw<TW>(a.bz, q), bx0 = rdw<TW>(a.bx, q), by0 = rdw<TW>(a.by, q);
+            double bz0 = rdw<TW>(a.bz, q), bx0 = rdw<TW>(a.bx, q), by0 = rdw<TW>(a.by, q);
+            if (a.bxm) {
+                bx0 += a.bxm[k];
+                by0 += a.bym[k];
+                bz0 += a.bzm;
+            }
             const double bsq0 = bx0 * bx0 + by0 * by0 + bz0 * bz0;
             const double xx = sqrt(u0 * u0 + v0 * v0) + g1 * sqrt(t0) + sqrt((bsq0 * f) / r0);
             const double xz = sqrt(w0 * w0) + g1 * sqrt(t0) + sqrt((bsq0 * f) / r0);
@@ -709,6 +747,48 @@ __global__ void com_shift_kernel(Geom g, double* u, double* v, const double* res
 int launch_com_shift(Geom g, double* u, double* v, const double* res, hipStream_t s)
 {
     hipLaunchKernelGGL(com_shift_kernel, dim3(nblk(g.n3)), dim3(NT), 0, s, g, u, v, res);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: FP32 fluctuation shadows of the FP64 state
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) shadow_kernel(Geom g, const double* u, const double* v, const double* w,
+                                                    const double* t, const double* r, float* su, float* sv,
+                                                    float* sw, float* st, float* sr, const double* t0,
+                                                    const double* r0)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= g.n3) return;
+    const int k = (int)(q / g.nxy);
+    su[q] = (float)u[q];
+    sv[q] = (float)v[q];
+    sw[q] = (float)w[q];
+    st[q] = (float)(t[q] - t0[k]);
+    sr[q] = (float)(r[q] - r0[k]);
+}
+
+int launch_shadow(Geom g, const double* u, const double* v, const double* w, const double* t, const double* r,
+                  float* su, float* sv, float* sw, float* st, float* sr, const double* t0, const double* r0,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(shadow_kernel, dim3(nblk(g.n3)), dim3(NT), 0, s, g, u, v, w, t, r, su, sv, sw, st, sr, t0,
+                       r0);
+    return hipGetLastError() == hipSuccess ? 0 : -22;
+}
+
+__global__ void __launch_bounds__(NT) add_mean_kernel(const float* src, double* dst, const double* mean, double sc,
+                                                      Geom g)
+{
+    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= g.n3) return;
+    const int k = (int)(q / g.nxy);
+    dst[q] = (double)src[q] + (mean ? mean[k] : 0.0) + sc;
+}
+
+int launch_add_mean(const float* src, double* dst, const double* mean, double sc, Geom g, hipStream_t s)
+{
+    hipLaunchKernelGGL(add_mean_kernel, dim3(nblk(g.n3)), dim3(NT), 0, s, src, dst, mean, sc, g);
     return hipGetLastError() == hipSuccess ? 0 : -22;
 }
 

@@ -76,12 +76,14 @@ struct MagArgs {
     int icutx, icuty;
     const double *akx1, *akx2, *aky1, *aky2;
     double *pol, *tor, *bxbar, *bybar;
-    double *poldot, *tordot, *bxbardot, *bybardot;   // newest history level
+    void *poldot, *tordot;        // newest history level (working precision)
+    double *bxbardot, *bybardot;  // newest history level of the means (FP64)
     void *exh, *eyh, *ezh;        // spectral EMF / B (double-halfcomplex)
     void *d, *cc, *dz;            // mag_deriv work
     void *polz;                   // bandj: d pol / dz
     void *bxh, *byh, *bzh, *jzh, *dybzh, *dxbzh;  // bandj spectral outputs
     const void *bx, *by;          // physical B for J^2
+    const double *bxm, *bym;      // mixed: means of bx, by (their d/dz enters J^2)
     void *jsq;
     int wf32;
 };
@@ -97,7 +99,8 @@ struct AB3Args {
     Geom g;
     double c0, c1, c2;
     double* var[7];           // u v w r t pol tor
-    double* hist[7][3];       // [field][level 1..3] physical slots
+    void* hist[7][3];         // [field][level 1..3] physical slots
+    int h32;                  // history arrays are float
     double* mvar[2];          // bxbar bybar
     double* mhist[2][3];
     int nfields;              // 5 (hydro only) or 7
@@ -115,6 +118,8 @@ struct BCArgs {
     Geom g;
     double theta;
     double *u, *v, *w, *t, *pol, *tor, *bxbar, *bybar;
+    float *su, *sv, *sw, *st;  // mixed: FP32 shadows of the rows set here, or null
+    const double* t0;          // mixed: background subtracted for st
     int hydro;   // boundary()
     int mag;     // mag_boundary()
 };
@@ -131,13 +136,25 @@ struct RedArgs {
     const void *bx, *by, *bz;
     const void *f0, *f1, *f2;   // generic inputs (minmax: f0; divb: f0+f1+f2)
     int f_f32;
+    const double* f0m;          // per-plane mean added to f0 (minmax), or null
+    double f0s;                 // scalar added to f0 (minmax)
     int wf32;                   // bx/by/bz are float
+    const double *bxm, *bym;    // mixed: means of bx, by (null: B holds them)
+    double bzm;                 // mixed: mean of bz
     double* partial;            // nz * 8
     double* result;             // 8 doubles
 };
 int launch_reduce(const RedArgs& a, hipStream_t s);
 // subtract result[0], result[1] from u, v (GlobalAverages side effect)
 int launch_com_shift(Geom g, double* u, double* v, const double* res, hipStream_t s);
+
+// ---- mixed precision: FP32 fluctuation shadows of the FP64 state ------------
+// su..sw = u..w, st = t - t0(z), sr = r - r0(z), rounded to float
+int launch_shadow(Geom g, const double* u, const double* v, const double* w, const double* t, const double* r,
+                  float* su, float* sv, float* sw, float* st, float* sr, const double* t0, const double* r0,
+                  hipStream_t s);
+// dst[k-plane] = (double)src + mean[k] + s (mean may be null)
+int launch_add_mean(const float* src, double* dst, const double* mean, double sc, Geom g, hipStream_t s);
 
 // ---- misc -------------------------------------------------------------------
 int launch_copy_cvt(const void* src, int src_f32, void* dst, int dst_f32, long n, hipStream_t s);
