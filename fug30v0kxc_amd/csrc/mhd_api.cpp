@@ -72,7 +72,7 @@ struct mhd_ctx {
     void* wk[W_COUNT];
     double *red_partial, *red_result, *d_bzbar;
     double* stage;     // device FP64 staging (one field)
-    int max_groups;    // resident workgroups for the persistent transform kernels
+    int ncu;           // compute units (persistent transform grids)
     int hzc;           // planes per thread of the column-walking hydro kernel
     double* hbuf;      // pinned host staging (one field)
 
@@ -219,7 +219,7 @@ SpecLaunch spec_base(mhd_ctx* c, int dir, int mode, bool f64 = false)
     if (f64) L.tw = dir == 0 ? c->twx64 : c->twy64;
     else L.tw = dir == 0 ? c->twx : c->twy;
     L.icut = dir == 0 ? c->icutx : c->icuty;
-    L.max_groups = c->max_groups;
+    L.ncu = c->ncu;
     return L;
 }
 
@@ -826,7 +826,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     set_constants(c);
     c->ring[0] = 0; c->ring[1] = 1; c->ring[2] = 2;
     c->sc.first = 1;
-    c->max_groups = 8 * ncu;   // enough resident workgroups to keep every CU busy
+    c->ncu = ncu;
     {
         // column-walk chunks: about 2048 workgroups of 256 columns, >= 8 planes each
         const long bx = (c->nxy + 255) / 256;

@@ -50,7 +50,8 @@ struct SpecLaunch {
     double cx;        // 2*pi/xmax (dir x) or 2*pi/ymax (dir y): mode m has wavenumber m*cx
     const void* tw;   // W_N^t table in the compute type
     int icut;         // dealias cut-off (n/2)/3
-    int max_groups;   // resident workgroups to launch (persistent loop); 0 = one per item
+    int ncu;          // compute units: the persistent grid is occupancy x ncu workgroups
+    int max_groups;   // optional further cap on the grid; 0 = none
 };
 
 // Returns 0 or a negative code (-20: unsupported length).

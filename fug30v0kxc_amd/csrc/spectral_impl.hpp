@@ -20,10 +20,31 @@
 // of each of its two rows per load.  Columns (dir=y): consecutive threads
 // belong to consecutive column pairs, so one load reads 2*B adjacent columns
 // of one row (2*B*sizeof(T) bytes).
+#include <type_traits>
+
 #include "fft_reg.hpp"
 #include "spectral.hpp"
 
 namespace mhd {
+
+// Adjacent pairs (i, i+1) with i even: the two columns of a y transform.
+// One 8-byte (float) or 16-byte (double) access instead of two.
+template <typename TC>
+using vec2_t = typename std::conditional<sizeof(TC) == 4, float2, double2>::type;
+template <typename TC>
+__device__ __forceinline__ cpx<TC> ld_pair(const TC* __restrict__ p, long i)
+{
+    const vec2_t<TC> v = *reinterpret_cast<const vec2_t<TC>*>(p + i);
+    return {(TC)v.x, (TC)v.y};
+}
+template <typename TC>
+__device__ __forceinline__ void st_pair(TC* __restrict__ p, long i, TC a, TC b)
+{
+    vec2_t<TC> v;
+    v.x = a;
+    v.y = b;
+    *reinterpret_cast<vec2_t<TC>*>(p + i) = v;
+}
 
 // Input loaders, fixed at compile time per launch.  Inputs have the compute
 // type TC, except LD_BG, which reads the FP64 state and subtracts a per-plane
@@ -53,6 +74,29 @@ struct Loader {
             if (J.mean[1]) m1 = (TC)J.mean[1][k];
             if (J.mean[2]) m2 = (TC)J.mean[2][k];
             if (J.mean[3]) m3 = (TC)J.mean[3][k];
+        }
+    }
+    // elements i and i+1 (i even)
+    __device__ __forceinline__ cpx<TC> pair(long i) const
+    {
+        if constexpr (LOP == LD_PLAIN) return ld_pair(a, i);
+        else if constexpr (LOP == LD_BG) {
+            const cpx<double> v = ld_pair(s64, i);
+            return {(TC)(v.x - bgk), (TC)(v.y - bgk)};
+        } else if constexpr (LOP == LD_PROD) {
+            const cpx<TC> x = ld_pair(a, i), y = ld_pair(b, i);
+            return {(scale * x.x) * y.x + scale * (x.x * m1 + m0 * y.x),
+                    (scale * x.y) * y.y + scale * (x.y * m1 + m0 * y.y)};
+        } else if constexpr (LOP == LD_MAXW) {
+            const cpx<TC> x = ld_pair(a, i), y = ld_pair(b, i), z = ld_pair(c, i);
+            return {scale * x.x * x.x - scale * y.x * y.x - scale * z.x * z.x +
+                        (TC)2 * scale * (m0 * x.x - m1 * y.x - m2 * z.x),
+                    scale * x.y * x.y - scale * y.y * y.y - scale * z.y * z.y +
+                        (TC)2 * scale * (m0 * x.y - m1 * y.y - m2 * z.y)};
+        } else {
+            const cpx<TC> x = ld_pair(a, i), y = ld_pair(b, i), z = ld_pair(c, i), w = ld_pair(d, i);
+            return {x.x * y.x - z.x * w.x + (x.x * m1 + m0 * y.x) - (z.x * m3 + m2 * w.x),
+                    x.y * y.y - z.y * w.y + (x.y * m1 + m0 * y.y) - (z.y * m3 + m2 * w.y)};
         }
     }
     __device__ __forceinline__ TC operator()(long i) const
@@ -183,7 +227,10 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
             for (int r = 0; r < R0; ++r) {
                 const int n = j + r * NB0;
                 dst[r] = {(TC)0, (TC)0};
-                if (ok && j < NB0) dst[r] = {src[ia + n * st], src[ib + n * st]};
+                if (ok && j < NB0) {
+                    if constexpr (DIR == 1) dst[r] = ld_pair(src, ia + n * st);
+                    else dst[r] = {src[ia + n * st], src[ib + n * st]};
+                }
             }
         } else {
             const Loader<TC, LOP> ld(J, ok ? k : 0);
@@ -191,7 +238,10 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
             for (int r = 0; r < R0; ++r) {
                 const int n = j + r * NB0;
                 dst[r] = {(TC)0, (TC)0};
-                if (ok && j < NB0) dst[r] = {ld(ia + n * st), ld(ib + n * st)};
+                if (ok && j < NB0) {
+                    if constexpr (DIR == 1) dst[r] = ld.pair(ia + n * st);
+                    else dst[r] = {ld(ia + n * st), ld(ib + n * st)};
+                }
             }
         }
     };
@@ -242,8 +292,12 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
 #pragma unroll
                 for (int r = 0; r < R0; ++r) {
                     const int n = j + r * NB0;
-                    out0[ia + n * st] = v[r].x * (TC)(1.0 / N) + add;
-                    out0[ib + n * st] = v[r].y * (TC)(1.0 / N) + add;
+                    const TC xa = v[r].x * (TC)(1.0 / N) + add, xb = v[r].y * (TC)(1.0 / N) + add;
+                    if constexpr (DIR == 1) st_pair(out0, ia + n * st, xa, xb);
+                    else {
+                        out0[ia + n * st] = xa;
+                        out0[ib + n * st] = xb;
+                    }
                 }
             }
         } else {
@@ -265,8 +319,11 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
                         TC a, b;
                         if (n <= N / 2) { a = (TC)0.5 * (zk.x + zn.x); b = (TC)0.5 * (zk.y + zn.y); }
                         else { a = (TC)0.5 * (zk.y - zn.y); b = (TC)0.5 * (zn.x - zk.x); }
-                        out0[ia + n * st] = a;
-                        out0[ib + n * st] = b;
+                        if constexpr (DIR == 1) st_pair(out0, ia + n * st, a, b);
+                        else {
+                            out0[ia + n * st] = a;
+                            out0[ib + n * st] = b;
+                        }
                     }
                 }
                 __syncthreads();
@@ -290,8 +347,11 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
 #pragma unroll
                         for (int r = 0; r < R0; ++r) {
                             const int n = j + r * NB0;
-                            out[ia + n * st] = v[r].x + add;
-                            out[ib + n * st] = v[r].y + add;
+                            if constexpr (DIR == 1) st_pair(out, ia + n * st, v[r].x + add, v[r].y + add);
+                            else {
+                                out[ia + n * st] = v[r].x + add;
+                                out[ib + n * st] = v[r].y + add;
+                            }
                         }
                     }
                 }
@@ -331,7 +391,15 @@ static int launch_one(const SpecLaunch& L, hipStream_t s)
     static_assert(lds <= 65536, "transform workgroup exceeds 64 KB of LDS");
     auto kern = spec_kernel<TC, PL, DIR, B, MODE, LOP>;
     const long items = Lines<PL, DIR, B>::groups(L.g) * L.njobs;
+    // persistent grid: the workgroups that fit on the device at once
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, NT, lds) != hipSuccess || n < 1) n = 1;
+        per_cu = n;
+    }
     long grid = items;
+    if (L.ncu > 0 && grid > (long)per_cu * L.ncu) grid = (long)per_cu * L.ncu;
     if (L.max_groups > 0 && grid > L.max_groups) grid = L.max_groups;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, s, L);
     return hipGetLastError() == hipSuccess ? 0 : -22;
