@@ -17,9 +17,9 @@ energy) are reported under "mixed".
 Prints one JSON line.  The roofline object covers the dominant kernel family
 (the x-direction spectral first+second derivative of T, u, v, w), timed with
 HIP events on the solver's stream over the timed steps.  cpu_baseline times
-the CPU oracle (a serial C restatement of the reference) on a bounded sample
-of the same workload: a 256x256x16 slab (same plane size, so the same
-per-cell transform cost), a few steps, one thread.
+the CPU oracle (a C restatement of the reference) on a bounded sample of the
+same workload: a few steps of the full box on all host threads, and one
+step on one thread.
 """
 import argparse
 import json
@@ -42,19 +42,31 @@ def workload_params():
     return reference_params(ra=1e6, q=1000.0, t_amp=1e-3, b_amp=1e-3)
 
 
-def cpu_baseline(params, nz=16, steps=6):
-    """Time the CPU oracle on a 256x256x`nz` slab (1 thread)."""
-    from oracle.pyoracle import Oracle
-    o = Oracle(NX, NY, nz, params)
+def cpu_baseline(params, steps=3, steps1=1):
+    """Time the CPU oracle (serial C restatement of the reference, OpenMP over
+    planes and points, results independent of the thread count) on the full
+    256x256x160 box: `steps` steps on all of this host's threads, then
+    `steps1` step on one thread, each after one untimed step."""
+    from oracle import pyoracle
+    o = pyoracle.Oracle(NX, NY, NZ, params)
     o.setup_static()
-    o.step(1, DT, 0)          # first step includes the Euler start-up
+    nthreads = pyoracle.threads()
+    o.step(1, DT, 0)          # the Euler start-up step, untimed
     t0 = time.perf_counter()
     o.step(steps, DT, 0)
     dt = time.perf_counter() - t0
-    return {"value": NX * NY * nz * steps / dt / 1e6, "unit": "Mcell-steps/s", "cores": 1,
+    pyoracle.set_threads(1)
+    t1 = time.perf_counter()
+    o.step(steps1, DT, 0)
+    dt1 = time.perf_counter() - t1
+    pyoracle.set_threads(nthreads)
+    cells = NX * NY * NZ
+    return {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": nthreads,
             "kind": "port",
-            "sample": f"CPU oracle (serial C restatement of the reference), {NX}x{NY}x{nz} slab, "
-                      f"{steps} FP64 AB3 steps after 1 untimed, 1 thread, {dt:.1f} s"}
+            "single_thread": round(cells * steps1 / dt1 / 1e6, 3),
+            "sample": f"CPU oracle (C restatement of the reference, FP64), full {NX}x{NY}x{NZ} box: "
+                      f"{steps} AB3 steps on {nthreads} threads ({dt:.1f} s) and {steps1} on 1 thread "
+                      f"({dt1:.1f} s), each after 1 untimed step"}
 
 
 def run(precision, steps, warmup, kernel):

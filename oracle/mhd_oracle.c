@@ -23,6 +23,15 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* OpenMP over planes / points for the CPU baseline.  Only loops whose
+ * iterations write disjoint elements are parallel; every reduction keeps its
+ * serial order, so results do not depend on the thread count. */
+#if defined(_OPENMP)
+#define ORC_PAR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PAR
+#endif
+
 /* ------------------------------------------------------------------------ */
 /* complex helpers and a mixed-radix (4,2,3,5,generic) DIT FFT              */
 /* ------------------------------------------------------------------------ */
@@ -149,6 +158,27 @@ static void small_dft(const fftplan* p, int r, cplx* t, cplx* out, int k, int m,
     }
 }
 
+/* per-thread scratch of the transforms and of the row operators */
+static _Thread_local cplx *tl_in, *tl_out, *tl_tmp;
+static _Thread_local double *tl_row, *tl_row2;
+static _Thread_local size_t tl_cap;
+static void tl_ensure(size_t n)
+{
+    if (n <= tl_cap) return;
+    tl_in = (cplx*)realloc(tl_in, sizeof(cplx) * n);
+    tl_out = (cplx*)realloc(tl_out, sizeof(cplx) * n);
+    tl_tmp = (cplx*)realloc(tl_tmp, sizeof(cplx) * n);
+    tl_row = (double*)realloc(tl_row, sizeof(double) * n);
+    tl_row2 = (double*)realloc(tl_row2, sizeof(double) * n);
+    tl_cap = n;
+}
+
+static double* orc_row(size_t n, int which)
+{
+    tl_ensure(n);
+    return which ? tl_row2 : tl_row;
+}
+
 static void fft_rec(const fftplan* p, int n, const cplx* in, long is, cplx* out,
                     int level, int sign)
 {
@@ -158,7 +188,7 @@ static void fft_rec(const fftplan* p, int n, const cplx* in, long is, cplx* out,
     for (int s = 0; s < r; s++)
         fft_rec(p, m, in + (long)s * is, is * r, out + (long)s * m, level + 1, sign);
     const long twstride = p->n / n;
-    cplx* t = p->tmp;
+    cplx* t = tl_tmp;
     for (int k = 0; k < m; k++) {
         for (int s = 0; s < r; s++) {
             cplx y = out[(long)s * m + k];
@@ -178,27 +208,31 @@ static void cfft(fftplan* p, const cplx* in, cplx* out, int sign)
 static void r2hc_strided(fftplan* p, const double* in, long istride, double* out, long ostride)
 {
     const int n = p->n;
-    for (int t = 0; t < n; t++) { p->buf_in[t].re = in[t * istride]; p->buf_in[t].im = 0.0; }
-    cfft(p, p->buf_in, p->buf_out, -1);
-    out[0] = p->buf_out[0].re;
-    for (int m = 1; m <= n / 2; m++) out[m * ostride] = p->buf_out[m].re;
-    for (int m = 1; m < (n + 1) / 2; m++) out[(n - m) * ostride] = p->buf_out[m].im;
+    tl_ensure((size_t)n);
+    cplx *bi = tl_in, *bo = tl_out;
+    for (int t = 0; t < n; t++) { bi[t].re = in[t * istride]; bi[t].im = 0.0; }
+    cfft(p, bi, bo, -1);
+    out[0] = bo[0].re;
+    for (int m = 1; m <= n / 2; m++) out[m * ostride] = bo[m].re;
+    for (int m = 1; m < (n + 1) / 2; m++) out[(n - m) * ostride] = bo[m].im;
 }
 
 /* FFTW-2 rfftw COMPLEX_TO_REAL (unnormalised) */
 static void hc2r_strided(fftplan* p, const double* in, long istride, double* out, long ostride)
 {
     const int n = p->n;
-    p->buf_in[0].re = in[0]; p->buf_in[0].im = 0.0;
+    tl_ensure((size_t)n);
+    cplx *bi = tl_in, *bo = tl_out;
+    bi[0].re = in[0]; bi[0].im = 0.0;
     for (int m = 1; m < (n + 1) / 2; m++) {
-        p->buf_in[m].re = in[m * istride];
-        p->buf_in[m].im = in[(n - m) * istride];
-        p->buf_in[n - m].re = p->buf_in[m].re;
-        p->buf_in[n - m].im = -p->buf_in[m].im;
+        bi[m].re = in[m * istride];
+        bi[m].im = in[(n - m) * istride];
+        bi[n - m].re = bi[m].re;
+        bi[n - m].im = -bi[m].im;
     }
-    if (n % 2 == 0) { p->buf_in[n / 2].re = in[(n / 2) * istride]; p->buf_in[n / 2].im = 0.0; }
-    cfft(p, p->buf_in, p->buf_out, +1);
-    for (int t = 0; t < n; t++) out[t * ostride] = p->buf_out[t].re;
+    if (n % 2 == 0) { bi[n / 2].re = in[(n / 2) * istride]; bi[n / 2].im = 0.0; }
+    cfft(p, bi, bo, +1);
+    for (int t = 0; t < n; t++) out[t * ostride] = bo[t].re;
 }
 
 void orc_r2hc(int n, const double* in, double* out) { r2hc_strided(get_plan(n), in, 1, out, 1); }
@@ -400,8 +434,12 @@ void orc_set_scalar(orc_ctx* c, int id, double v)
 void orc_d1bydx(orc_ctx* c, const double* fin, double* fout)
 {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    double* t = c->rowbuf;
-    for (int k = 0; k < nz; k++)
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int j = 0; j < ny; j++) {
             double* o = fout + IDX(c, 0, j, k);
             r2hc_strided(c->px, fin + IDX(c, 0, j, k), 1, o, 1);
@@ -415,14 +453,19 @@ void orc_d1bydx(orc_ctx* c, const double* fin, double* fout)
             hc2r_strided(c->px, o, 1, t, 1);
             for (int i = 0; i < nx; i++) o[i] = t[i] / (double)nx;
         }
+    }
 }
 
 /* src/fft.F90:329-398 */
 void orc_d1bydy(orc_ctx* c, const double* fin, double* fout)
 {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    double* t = c->rowbuf;
-    for (int k = 0; k < nz; k++)
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int i = 0; i < nx; i++) {
             double* o = fout + IDX(c, i, 0, k);
             r2hc_strided(c->py, fin + IDX(c, i, 0, k), nx, o, nx);
@@ -436,14 +479,19 @@ void orc_d1bydy(orc_ctx* c, const double* fin, double* fout)
             hc2r_strided(c->py, o, nx, t, 1);
             for (int j = 0; j < ny; j++) o[(size_t)j * nx] = t[j] / (double)ny;
         }
+    }
 }
 
 /* src/fft.F90:403-489 */
 void orc_d2bydx(orc_ctx* c, const double* fin, double* f1, double* f2)
 {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    double* t = c->rowbuf;
-    for (int k = 0; k < nz; k++)
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int j = 0; j < ny; j++) {
             double* a = f1 + IDX(c, 0, j, k);
             double* b = f2 + IDX(c, 0, j, k);
@@ -463,15 +511,19 @@ void orc_d2bydx(orc_ctx* c, const double* fin, double* f1, double* f2)
             hc2r_strided(c->px, b, 1, t, 1);
             for (int i = 0; i < nx; i++) b[i] = t[i] / (double)nx;
         }
+    }
 }
 
 /* src/fft.F90:495-581 */
 void orc_d2bydy(orc_ctx* c, const double* fin, double* f1, double* f2)
 {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    double* t = c->rowbuf;
-    double* s = c->rowbuf2;
-    for (int k = 0; k < nz; k++)
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int i = 0; i < nx; i++) {
             double* a = f1 + IDX(c, i, 0, k);
             double* b = f2 + IDX(c, i, 0, k);
@@ -491,14 +543,19 @@ void orc_d2bydy(orc_ctx* c, const double* fin, double* f1, double* f2)
             hc2r_strided(c->py, b, nx, t, 1);
             for (int j = 0; j < ny; j++) b[(size_t)j * nx] = t[j] / (double)ny;
         }
+    }
 }
 
 /* src/fft.F90:586-682 */
 void orc_dealias(orc_ctx* c, double* z)
 {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    double* t = c->rowbuf;
-    for (int k = 0; k < nz; k++)
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int j = 0; j < ny; j++) {
             double* o = z + IDX(c, 0, j, k);
             r2hc_strided(c->px, o, 1, t, 1);
@@ -508,7 +565,13 @@ void orc_dealias(orc_ctx* c, double* z)
             hc2r_strided(c->px, o, 1, t, 1);
             for (int i = 0; i < nx; i++) o[i] = t[i] / (double)nx;
         }
-    for (int k = 0; k < nz; k++)
+    }
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int i = 0; i < nx; i++) {
             double* o = z + IDX(c, i, 0, k);
             r2hc_strided(c->py, o, nx, t, 1);
@@ -521,42 +584,65 @@ void orc_dealias(orc_ctx* c, double* z)
             hc2r_strided(c->py, o, nx, t, 1);
             for (int j = 0; j < ny; j++) o[(size_t)j * nx] = t[j] / (double)ny;
         }
+    }
 }
 
 /* src/fft.F90:687-735 */
 void orc_forwardfft(orc_ctx* c, const double* fin, double* fout)
 {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    double* t = c->rowbuf;
-    for (int k = 0; k < nz; k++)
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int j = 0; j < ny; j++)
             r2hc_strided(c->px, fin + IDX(c, 0, j, k), 1, fout + IDX(c, 0, j, k), 1);
-    for (int k = 0; k < nz; k++)
+    }
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int i = 0; i < nx; i++) {
             double* o = fout + IDX(c, i, 0, k);
             r2hc_strided(c->py, o, nx, t, 1);
             for (int j = 0; j < ny; j++) o[(size_t)j * nx] = t[j];
         }
+    }
 }
 
 /* src/fft.F90:740-799 */
 void orc_inversefft(orc_ctx* c, const double* fin, double* fout)
 {
     const int nx = c->nx, ny = c->ny, nz = c->nz;
-    double* t = c->rowbuf;
     if (fout != fin) memcpy(fout, fin, sizeof(double) * c->n3);
-    for (int k = 0; k < nz; k++)
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int j = 0; j < ny; j++) {
             double* o = fout + IDX(c, 0, j, k);
             hc2r_strided(c->px, o, 1, t, 1);
             for (int i = 0; i < nx; i++) o[i] = t[i] / (double)nx;
         }
-    for (int k = 0; k < nz; k++)
+    }
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* t = orc_row((size_t)(nx > ny ? nx : ny), 0);
+        double* s = orc_row((size_t)(nx > ny ? nx : ny), 1);
+        (void)t;
+        (void)s;
         for (int i = 0; i < nx; i++) {
             double* o = fout + IDX(c, i, 0, k);
             hc2r_strided(c->py, o, nx, t, 1);
             for (int j = 0; j < ny; j++) o[(size_t)j * nx] = t[j] / (double)ny;
         }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -660,6 +746,7 @@ int orc_d1bydz(orc_ctx* c, const double* fin, double* fout, int fl)
         e |= topdz_low(h, &fout[m + q], F[q], F[m + q], F[2 * m + q], F[3 * m + q], fl);
     }
     if (e) return 7;
+    ORC_PAR
     for (int k = 2; k <= nz - 3; k++)
         for (size_t q = 0; q < m; q++)
             fout[k * m + q] = d1int(h, F[(k - 2) * m + q], F[(k - 1) * m + q],
@@ -688,6 +775,7 @@ int orc_d1upbydz(orc_ctx* c, const double* fin, double* fout, int fl, const doub
                                 F[4 * m + q], F[5 * m + q]);
     }
     if (e) return 7;
+    ORC_PAR
     for (int k = 3; k <= nz - 4; k++)
         for (size_t q = 0; q < m; q++)
             fout[k * m + q] = dupint(h, w[k * m + q], F[(k - 3) * m + q], F[(k - 2) * m + q],
@@ -718,6 +806,7 @@ int orc_d2bydz(orc_ctx* c, const double* fin, double* fout, int fl)
                         F[4 * m + q], F[5 * m + q], fl);
     }
     if (e) return 7;
+    ORC_PAR
     for (int k = 2; k <= nz - 3; k++)
         for (size_t q = 0; q < m; q++)
             fout[k * m + q] = d2int(s, F[(k - 2) * m + q], F[(k - 1) * m + q], F[k * m + q],
@@ -928,6 +1017,7 @@ static int jsq_tail(orc_ctx* c)
     int e = orc_d1bydz(c, c->fld[ORC_BY], wk1, 5);
     e |= orc_d1bydz(c, c->fld[ORC_BX], wk2, 5);
     if (e) return 7;
+    ORC_PAR
     for (size_t q = 0; q < n3; q++)
         jsq[q] = ez[q] * ez[q] + ((ex[q] - wk1[q]) * (ex[q] - wk1[q]))
                  + ((wk2[q] - ey[q]) * (wk2[q] - ey[q]));
@@ -973,6 +1063,7 @@ int orc_mag_init(orc_ctx* c)
     /* FFT(dBy/dx) in wk3, FFT(dBx/dy) in wk4, wk3 = FFT(jz) */
     spec_ddx(c, ey, wk3);
     spec_ddy(c, ex, wk4);
+    ORC_PAR
     for (size_t q = 0; q < c->n3; q++) wk3[q] = wk3[q] - wk4[q];
     for (int k = 0; k < nz; k++) {
         tor[IDX(c, 0, 0, k)] = 0.0;
@@ -1012,6 +1103,7 @@ int orc_mag_deriv(orc_ctx* c)
     double* w2d = (double*)malloc(sizeof(double) * (size_t)nz);
     int e = 0;
 
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         wk1[q] = v[q] * bz[q] - w[q] * by[q];
         wk2[q] = w[q] * bx[q] - u[q] * bz[q];
@@ -1037,6 +1129,7 @@ int orc_mag_deriv(orc_ctx* c)
     spec_ddx(c, ey, wk2);
     spec_ddy(c, ex, wk3);
     spec_ddy(c, ey, wk4);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         wk2[q] = wk3[q] - wk2[q];
         wk1[q] = wk1[q] + wk4[q];
@@ -1131,6 +1224,7 @@ int orc_hydro_deriv(orc_ctx* c)
     /* 33-53 */
     orc_d1bydx(c, r, wk1);
     orc_d1bydy(c, r, wk2);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         rdot[q] = -u[q] * wk1[q] - v[q] * wk2[q];
         ex[q] = -t[q] * wk1[q];
@@ -1164,6 +1258,7 @@ int orc_hydro_deriv(orc_ctx* c)
     orc_d1bydy(c, v, wk2);
     e |= orc_d1upbydz(c, w, wk3, 1, w);
     e |= orc_d1bydz(c, w, wk4, 1);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         double z1 = wk1[q], z2 = wk2[q], z3 = wk3[q], z4 = wk4[q];
         double tt = t[q], rr = r[q];
@@ -1191,6 +1286,7 @@ int orc_hydro_deriv(orc_ctx* c)
     e |= orc_d1bydz(c, wk4, wk3, 5);
     orc_d1bydx(c, v, wk2);
     orc_d1bydy(c, u, wk1);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         udot[q] = udot[q] + ex[q];
         vdot[q] = vdot[q] + ey[q];
@@ -1209,6 +1305,7 @@ int orc_hydro_deriv(orc_ctx* c)
     orc_d1bydy(c, wk3, wk1);
     orc_d1bydx(c, w, wk3);
     e |= orc_d1upbydz(c, u, wk4, 2, w);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         udot[q] = udot[q] + wk1[q];
         vdot[q] = vdot[q] + wk2[q];
@@ -1225,6 +1322,7 @@ int orc_hydro_deriv(orc_ctx* c)
     orc_d1bydx(c, wk2, wk4);
     orc_d1bydy(c, w, wk1);
     e |= orc_d1upbydz(c, v, wk2, 2, w);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         double rr = r[q];
         udot[q] = udot[q] + wk3[q];
@@ -1243,6 +1341,7 @@ int orc_hydro_deriv(orc_ctx* c)
     /* 320-338 */
     orc_d1bydy(c, wk3, wk1);
     e |= orc_d1bydz(c, wk4, wk2, 5);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         vdot[q] = vdot[q] + wk2[q];
         wdot[q] = wdot[q] + wk1[q];
@@ -1250,6 +1349,7 @@ int orc_hydro_deriv(orc_ctx* c)
     /* 349-367 */
     e |= orc_d2bydz(c, u, wk1, 3);
     e |= orc_d2bydz(c, v, wk2, 3);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         udot[q] = udot[q] + c->sck * wk1[q];
         vdot[q] = vdot[q] + c->sck * wk2[q];
@@ -1258,6 +1358,7 @@ int orc_hydro_deriv(orc_ctx* c)
     orc_d2bydx(c, u, ex, wk1);
     orc_d2bydx(c, v, ey, wk2);
     orc_d2bydx(c, w, ez, wk3);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         udot[q] = udot[q] + c->sck * wk1[q];
         vdot[q] = vdot[q] + c->sck * wk2[q];
@@ -1267,6 +1368,7 @@ int orc_hydro_deriv(orc_ctx* c)
     orc_d2bydy(c, u, ex, wk1);
     orc_d2bydy(c, v, ey, wk2);
     orc_d2bydy(c, w, ez, wk3);
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         udot[q] = udot[q] + c->sck * wk1[q];
         vdot[q] = vdot[q] + c->sck * wk2[q];
@@ -1287,11 +1389,13 @@ int orc_hydro_deriv(orc_ctx* c)
             wk2[L + p] = wdot[L + p] / t[L + p];
         }
     }
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         rdot[q] = rdot[q] - w[q] * wk1[q];
         wdot[q] = wdot[q] - t[q] * wk2[q];
     }
     /* 487-497 */
+    ORC_PAR
     for (size_t q = 0; q < n3; q++) {
         double rr = 1.0 / (r[q]);
         tdot[q] = tdot[q] * rr;
@@ -1381,6 +1485,7 @@ void orc_global_averages(orc_ctx* c, double out[4])
     double ubar = urbar / rbar, vbar = vrbar / rbar;
     menergy = menergy * c->xzn;
     kenergy = kenergy * c->xzn;
+    ORC_PAR
     for (size_t q = 0; q < c->n3; q++) {
         u[q] = u[q] - ubar;
         v[q] = v[q] - vbar;
@@ -1526,6 +1631,7 @@ int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag)
         orc_ab3_coeffs(&c->first, &c->second, dt, &c->dt1, &c->dt2, cf);
         const double c0 = cf[0], c1 = cf[1], c2 = cf[2];
         /* 413-477 */
+        ORC_PAR
         for (size_t q = 0; q < n3; q++) {
             for (int fi = 0; fi < 7; fi++) {
                 double* d = dots[fi];
@@ -1589,3 +1695,13 @@ int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag)
     }
     return 0;
 }
+
+/* threads of the OpenMP loops (1 when built without OpenMP) */
+#if defined(_OPENMP)
+#include <omp.h>
+int orc_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+#else
+int orc_threads(void) { return 1; }
+void orc_set_threads(int n) { (void)n; }
+#endif

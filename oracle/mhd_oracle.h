@@ -118,4 +118,8 @@ int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag);
 #ifdef __cplusplus
 }
 #endif
+/* OpenMP threads used by the plane / point loops (results do not depend on it) */
+int orc_threads(void);
+void orc_set_threads(int n);
+
 #endif

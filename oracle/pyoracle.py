@@ -50,6 +50,9 @@ def lib():
         L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.orc_destroy.argtypes = [vp]
+        L.orc_threads.restype = ctypes.c_int
+        L.orc_threads.argtypes = []
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_field.restype = dp
         L.orc_field.argtypes = [vp, ctypes.c_int]
         L.orc_scalar.restype = ctypes.c_double
@@ -141,6 +144,15 @@ def ab3_coeffs(first, second, dt, dt1, dt2):
     lib().orc_ab3_coeffs(ctypes.byref(f), ctypes.byref(s), dt, ctypes.byref(a), ctypes.byref(b),
                          _p(c))
     return c[0], c[1], c[2], f.value, s.value, a.value, b.value
+
+
+def threads():
+    """OpenMP threads of the oracle's plane / point loops."""
+    return lib().orc_threads()
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
 
 
 class OracleError(RuntimeError):
