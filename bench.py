@@ -92,6 +92,19 @@ def run(precision, steps, warmup, kernel):
             "device_bytes": dev_bytes}
 
 
+def measured_traffic(precision, kernel):
+    """HBM bytes per xderiv2 launch from the committed FETCH_SIZE / WRITE_SIZE
+    summary (profiles/hbm_bytes_256x256x160.md), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_bytes_256x256x160.md")
+    if kernel != "xderiv2" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == precision:
+            return int(f[2])
+    return None
+
+
 def roofline_of(r, kernel, elem):
     cells = NX * NY * NZ
     # algorithmic bytes of one launch: 4 fields, 1 read + 2 writes each
@@ -99,7 +112,8 @@ def roofline_of(r, kernel, elem):
     avg_ms = r["kms"] / max(r["klaunch"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic("fp64" if elem == 8 else "mixed", kernel),
             "kernel": f"spec_kernel {kernel} (4 fields: d/dx and d2/dx2 of T,u,v,w)",
             "avg_launch_ms": round(avg_ms, 4), "launches": r["klaunch"],
             "bytes_per_launch": bytes_per_launch}
@@ -116,7 +130,19 @@ def main():
                          "(e.g. for a kernel profile of one mode)")
     ap.add_argument("--kernel", default="xderiv2", choices=("xderiv2", "yderiv2"),
                     help="kernel family for the roofline object")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 27                      # 2^27 doubles = 1 GiB read + 1 GiB written
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 3}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the whole box sits on one MI355X, see the sharding design notes)", file=sys.stderr)
