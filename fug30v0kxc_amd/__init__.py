@@ -22,7 +22,7 @@ from . import _abi
 from ._abi import (FIELD_IDS, HISTORY_FIELDS, KERNELS, MHD_PREC_FP64, MHD_PREC_MIXED, OPS,
                    VECTOR_FIELDS, MhdDiag, MhdParams, MhdScalars)
 
-__all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "write_dump", "read_dump", "MHD_PREC_FP64",
+__all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "write_dump", "read_dump", "expand", "box_filter", "MHD_PREC_FP64",
            "MHD_PREC_MIXED"]
 
 
@@ -102,6 +102,32 @@ def read_dump(path):
     meta = {n: getattr(m, n) for n, _ in _abi.MhdDump._fields_ if n != "history"}
     meta["history"] = m.history.decode(errors="replace")
     return dict(zip(DUMP_FIELDS, arrs)), meta
+
+
+def expand(field):
+    """The reference's expand (interpolate/fft.F90:832-927) on the GPU: a
+    (nz, ny, nx) field -> (nz, 2ny, 2nx) by spectral zero-padding."""
+    a = _f64(field)
+    nz, ny, nx = a.shape
+    out = np.empty((nz, 2 * ny, 2 * nx))
+    code = _abi.load().mhd_expand(nx, ny, nz, _dp(a), _dp(out))
+    if code:
+        raise MhdError(code, "expand")
+    return out
+
+
+def box_filter(field, factor):
+    """Mean over factor^3 blocks: (nz, ny, nx) -> (nz, ny, nx) // factor."""
+    a = _f64(field)
+    nz, ny, nx = a.shape
+    f = int(factor)
+    if f < 1:
+        raise ValueError("factor must be >= 1")
+    out = np.empty((max(nz // f, 0), max(ny // f, 0), max(nx // f, 0)))
+    code = _abi.load().mhd_box_filter(nx, ny, nz, f, _dp(a), _dp(out))
+    if code:
+        raise MhdError(code, "box_filter")
+    return out
 
 
 class Solver:

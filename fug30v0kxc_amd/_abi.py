@@ -115,6 +115,8 @@ def load():
         "mhd_device_bytes": (ctypes.c_long, [vp]),
         "mhd_write_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp), i]),
         "mhd_read_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp)]),
+        "mhd_expand": (i, [i, i, i, dp, dp]),
+        "mhd_box_filter": (i, [i, i, i, i, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

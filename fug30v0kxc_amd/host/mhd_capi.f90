@@ -112,6 +112,18 @@ module mhd_capi
       type(c_ptr) :: fields(8)
     end function
 
+    integer(c_int) function mhd_expand(nx, ny, nz, fin, fout) bind(C)
+      import :: c_int, c_double
+      integer(c_int), value :: nx, ny, nz
+      real(c_double) :: fin(*), fout(*)
+    end function
+
+    integer(c_int) function mhd_box_filter(nx, ny, nz, f, fin, fout) bind(C)
+      import :: c_int, c_double
+      integer(c_int), value :: nx, ny, nz, f
+      real(c_double) :: fin(*), fout(*)
+    end function
+
     type(c_ptr) function mhd_strerror(code) bind(C)
       import :: c_int, c_ptr
       integer(c_int), value :: code

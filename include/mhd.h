@@ -175,6 +175,19 @@ typedef struct mhd_dump {
 int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const fields[8], int iprec);
 int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8]);
 
+/* ---- resampling and coarse-graining (host arrays in, host arrays out) ---- */
+/* expand (interpolate/fft.F90:832-927, called per field at
+ * interpolate/main.F90:299-306): a (nx,ny,nz) field becomes (2nx,2ny,nz) by
+ * zero-padding the spectrum of every plane, the coarse Nyquist coefficient
+ * kept as a cosine of the fine grid.  Codes: -1/-2 odd nx/ny, -20 a length of
+ * nx, 2nx, ny, 2ny without a transform kernel. */
+int mhd_expand(int nx, int ny, int nz, const double* in, double* out);
+/* Block box filter of factor f (the closure data path: filtered fine fields
+ * as coarse targets): out(I,J,K) = mean of in over the f x f x f block at
+ * (fI, fJ, fK); (nx,ny,nz) -> (nx/f,ny/f,nz/f).  No counterpart in the
+ * reference.  MHD_E_ARG when f does not divide nx, ny and nz. */
+int mhd_box_filter(int nx, int ny, int nz, int f, const double* in, double* out);
+
 /* Waits for all work of the context; returns any pending device error. */
 int mhd_sync(mhd_ctx_t* ctx);
 /* Bytes of device memory held by the context. */

@@ -1705,3 +1705,79 @@ void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 int orc_threads(void) { return 1; }
 void orc_set_threads(int n) { (void)n; }
 #endif
+
+/* ------------------------------------------------------------------------ */
+/* spectral resampling and box filter                                       */
+/* ------------------------------------------------------------------------ */
+
+/* expand, interpolate/fft.F90:832-927: each plane of fin (nx, ny) becomes
+ * fout (2nx, 2ny) by zero-padding its spectrum.  Along x: forward r2hc of
+ * length nx; the 2nx-long halfcomplex line takes r_0, the r_i and i_i of
+ * i = 1..nx/2-1 (at i and 2nx-i) and the coarse Nyquist r_{nx/2} at index
+ * nx/2 as an ordinary cosine coefficient (lines 879-887); inverse hc2r of
+ * length 2nx divided by nx (889-896).  Then the same along y (898-925). */
+int orc_expand(int nx, int ny, int nz, const double* fin, double* fout)
+{
+    if (nx % 2 || ny % 2 || nx < 2 || ny < 2 || nz < 1) return -1;
+    const int nx2 = 2 * nx, ny2 = 2 * ny;
+    fftplan *px = get_plan(nx), *px2 = get_plan(nx2), *py = get_plan(ny), *py2 = get_plan(ny2);
+    double* mid = (double*)calloc((size_t)nx2 * ny * nz, sizeof(double));
+    if (!mid) return -5;
+    ORC_PAR
+    for (int k = 0; k < nz; k++) {
+        double* h = orc_row((size_t)(nx2 > ny2 ? nx2 : ny2), 0);
+        double* q = orc_row((size_t)(nx2 > ny2 ? nx2 : ny2), 1);
+        for (int j = 0; j < ny; j++) {
+            r2hc_strided(px, fin + ((size_t)k * ny + j) * nx, 1, h, 1);
+            for (int i = 0; i < nx2; i++) q[i] = 0.0;
+            q[0] = h[0];
+            for (int i = 1; i <= nx / 2 - 1; i++) {
+                q[i] = h[i];
+                q[nx2 - i] = h[nx - i];
+            }
+            q[nx / 2] = h[nx / 2];
+            double* o = mid + ((size_t)k * ny + j) * nx2;
+            hc2r_strided(px2, q, 1, o, 1);
+            for (int i = 0; i < nx2; i++) o[i] = o[i] / (double)nx;
+        }
+        for (int i = 0; i < nx2; i++) {
+            const double* col = mid + (size_t)k * ny * nx2 + i;
+            r2hc_strided(py, col, nx2, h, 1);
+            for (int j = 0; j < ny2; j++) q[j] = 0.0;
+            q[0] = h[0];
+            for (int j = 1; j <= ny / 2 - 1; j++) {
+                q[j] = h[j];
+                q[ny2 - j] = h[ny - j];
+            }
+            q[ny / 2] = h[ny / 2];
+            double* o = fout + (size_t)k * ny2 * nx2 + i;
+            hc2r_strided(py2, q, 1, o, nx2);
+            for (int j = 0; j < ny2; j++) o[(size_t)j * nx2] = o[(size_t)j * nx2] / (double)ny;
+        }
+    }
+    free(mid);
+    return 0;
+}
+
+/* Box filter of factor f for coarse-grained data: coarse point (I, J, K) is
+ * the mean of the fine block i in [fI, fI+f), j in [fJ, fJ+f), k in [fK,
+ * fK+f) of a (nx, ny, nz) field, giving (nx/f, ny/f, nz/f).  Summed over k,
+ * then j, then i, and divided by f^3 once. */
+int orc_box_filter(int nx, int ny, int nz, int f, const double* fin, double* fout)
+{
+    if (f < 1 || nx % f || ny % f || nz % f) return -1;
+    const int cx = nx / f, cy = ny / f, cz = nz / f;
+    const double inv = 1.0 / ((double)f * (double)f * (double)f);
+    ORC_PAR
+    for (int K = 0; K < cz; K++)
+        for (int J = 0; J < cy; J++)
+            for (int I = 0; I < cx; I++) {
+                double s = 0.0;
+                for (int kk = 0; kk < f; kk++)
+                    for (int jj = 0; jj < f; jj++)
+                        for (int ii = 0; ii < f; ii++)
+                            s += fin[((size_t)(f * K + kk) * ny + (f * J + jj)) * nx + (f * I + ii)];
+                fout[((size_t)K * cy + J) * cx + I] = s * inv;
+            }
+    return 0;
+}

@@ -122,4 +122,9 @@ int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag);
 int orc_threads(void);
 void orc_set_threads(int n);
 
+/* expand (interpolate/fft.F90:832-927): (nx,ny,nz) -> (2nx,2ny,nz); 0 or -1 / -5 */
+int orc_expand(int nx, int ny, int nz, const double* fin, double* fout);
+/* f-fold block box filter: (nx,ny,nz) -> (nx/f,ny/f,nz/f); 0 or -1 */
+int orc_box_filter(int nx, int ny, int nz, int f, const double* fin, double* fout);
+
 #endif

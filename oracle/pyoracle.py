@@ -53,6 +53,9 @@ def lib():
         L.orc_threads.restype = ctypes.c_int
         L.orc_threads.argtypes = []
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        ci = ctypes.c_int
+        L.orc_expand.argtypes = [ci, ci, ci, dp, dp]
+        L.orc_box_filter.argtypes = [ci, ci, ci, ci, dp, dp]
         L.orc_field.restype = dp
         L.orc_field.argtypes = [vp, ctypes.c_int]
         L.orc_scalar.restype = ctypes.c_double
@@ -153,6 +156,28 @@ def threads():
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def expand(f):
+    """expand of a (nz, ny, nx) field -> (nz, 2ny, 2nx)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nz, ny, nx = f.shape
+    out = np.zeros((nz, 2 * ny, 2 * nx))
+    e = lib().orc_expand(nx, ny, nz, _p(f), _p(out))
+    if e:
+        raise OracleError(e, "orc_expand")
+    return out
+
+
+def box_filter(f, factor):
+    """factor-fold block average of a (nz, ny, nx) field."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    nz, ny, nx = f.shape
+    out = np.zeros((nz // factor, ny // factor, nx // factor))
+    e = lib().orc_box_filter(nx, ny, nz, int(factor), _p(f), _p(out))
+    if e:
+        raise OracleError(e, "orc_box_filter")
+    return out
 
 
 class OracleError(RuntimeError):

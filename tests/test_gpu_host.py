@@ -19,6 +19,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "fug30v0kxc_amd", "host", "p3d_gpu")
+INTERP = os.path.join(ROOT, "fug30v0kxc_amd", "host", "interp_gpu")
 NAMES = {"r": "R", "t": "T", "u": "U", "v": "V", "w": "W", "bx": "Bx", "by": "By", "bz": "Bz"}
 NX, NY, NZ = 16, 16, 12
 DT = 1e-4
@@ -123,3 +124,26 @@ def test_bad_input_fails_loudly(tmp_path):
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "ERROR" in r.stdout
+
+
+def test_interpolation_tool(tmp_path):
+    """interp_gpu: restart.nc (nx, ny, nz) -> output.nc (2nx, 2ny, nz), each
+    field expanded as the reference's interpolate/main.F90:299-306."""
+    if not os.path.exists(INTERP):
+        pytest.skip("interp_gpu not built")
+    import fug30v0kxc_amd as fa
+    from oracle import pyoracle
+    rng = np.random.default_rng(5)
+    shape = (6, 12, 16)
+    fields = {n: rng.standard_normal(shape) + (2.0 if n in ("r", "t") else 0.0) for n in fa.DUMP_FIELDS}
+    fa.write_dump(tmp_path / "restart.nc", fields, iteration=42, time=1.25, timestep=1e-4,
+                  params=reference_params(), iprec=8)
+    print(f"[host] running interp_gpu in {tmp_path}", flush=True)
+    r = subprocess.run([INTERP], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got, meta = dump(tmp_path / "output.nc")
+    assert int(meta["iteration"]) == 42 and meta["itemsize"] == 8
+    for n in fa.DUMP_FIELDS:
+        want = pyoracle.expand(fields[n])
+        assert got[n].shape == (6, 24, 32)
+        assert np.abs(got[n] - want).max() <= 1e-12 * np.abs(want).max(), n
