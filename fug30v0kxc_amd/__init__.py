@@ -20,6 +20,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (FIELD_IDS, HISTORY_FIELDS, KERNELS, MHD_PREC_FP64, MHD_PREC_MIXED, OPS,
+                   OPT_CONSTFLUX, OPT_DYNAMO, OPT_KINEMATIC, OPT_PERFECT, OPT_RADBC,
                    VECTOR_FIELDS, MhdDiag, MhdParams, MhdScalars)
 
 __all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "write_dump", "read_dump", "expand", "box_filter", "MHD_PREC_FP64",

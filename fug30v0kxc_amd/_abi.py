@@ -22,6 +22,9 @@ MHD_E_SIZE = -20
 MHD_E_DEVICE = -30
 MHD_E_STATE = -40
 
+# MHD_OPT_* switches of mhd_params_t.opts
+OPT_RADBC, OPT_CONSTFLUX, OPT_PERFECT, OPT_KINEMATIC, OPT_DYNAMO = 1, 2, 4, 8, 16
+
 MHD_PREC_FP64 = 0
 MHD_PREC_MIXED = 1
 
@@ -43,7 +46,7 @@ KERNELS = {"none": 0, "xderiv2": 1, "yderiv2": 2, "zderiv": 3, "hydro": 4, "ab3"
 class MhdParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("gamma", "sigma", "theta", "ck", "cm", "f", "tayl", "psi", "zeta",
-                 "xmax", "ymax", "sf", "t_amp", "b_amp")]
+                 "xmax", "ymax", "sf", "t_amp", "b_amp")] + [("opts", ctypes.c_int)]
 
 
 class MhdScalars(ctypes.Structure):

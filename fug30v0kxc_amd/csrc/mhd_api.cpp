@@ -144,7 +144,8 @@ void set_constants(mhd_ctx* c)
     h.dtdiff0z = 0.2 * c->dz * c->dz / p.gamma / p.ck * (1.0 + p.theta / 7.0);
     h.ho12 = (double)(c->nz - 1) / 12.0;
     h.sc1 = (double)((c->nz - 1) * (c->nz - 1)) / 12.0;
-    h.mag = (p.f != 0.0);
+    // the field evolves when F != 0 or for the kinematic problem (src/main.F90:355-375)
+    h.mag = (p.f != 0.0) || (p.opts & MHD_OPT_KINEMATIC);
 }
 
 // src/fft.F90:171-221
@@ -560,6 +561,10 @@ int run_mag_boundary(mhd_ctx* c, int hydro, int mag)
     b.u = c->u; b.v = c->v; b.w = c->w; b.t = c->t;
     b.pol = c->pol; b.tor = c->tor; b.bxbar = c->bxbar; b.bybar = c->bybar;
     b.su = c->su; b.sv = c->sv; b.sw = c->sw; b.st = c->st; b.t0 = c->t0;
+    b.radbc = (c->p.opts & MHD_OPT_RADBC) != 0;
+    b.constflux = (c->p.opts & MHD_OPT_CONSTFLUX) != 0;
+    b.perfect = (c->p.opts & MHD_OPT_PERFECT) != 0;
+    b.dz = c->dz;
     b.hydro = hydro;
     b.mag = mag;
     return kerr(launch_boundary(b, c->s));
@@ -579,6 +584,34 @@ int run_mag_init(mhd_ctx* c)
     const int wf = c->wf32;
     MagArgs m = margs(c);
     const bool mx = c->su != nullptr;
+    if (c->p.opts & MHD_OPT_DYNAMO) {
+        // src/magnetic.f90:29-57: the seed field replaces B (1-based i, j in the cosines)
+        const int nx = c->nx, ny = c->ny, nz = c->nz;
+        const double seed = 0.003, tx = 8.0 * std::atan(1.0) / (double)nx, ty = 8.0 * std::atan(1.0) / (double)ny;
+        const bool pc = (c->p.opts & MHD_OPT_PERFECT) != 0;
+        std::vector<double> f(c->n3);
+        void* dst[3] = {c->bx, c->by, c->bz};
+        for (int comp = 0; comp < 3; ++comp) {
+            for (int k = 0; k < nz; ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        double v = 0.0;
+                        if (pc && comp == 0) v = seed * std::cos(ty * (double)(j + 1));
+                        if (pc && comp == 1) v = seed * std::cos(tx * (double)(i + 1));
+                        if (!pc && comp == 2) v = seed * std::cos(tx * (double)(i + 1)) * std::cos(ty * (double)(j + 1));
+                        f[((long)k * ny + j) * nx + i] = v;
+                    }
+            MHD_HIP_CHECK(hipMemcpy(c->stage, f.data(), c->n3 * sizeof(double), hipMemcpyHostToDevice));
+            TRY(kerr(launch_copy_cvt(c->stage, 0, dst[comp], c->wf32, c->n3, c->s)));
+        }
+        if (mx) {
+            // the seed has no plane means
+            TRY(kerr(launch_fill(c->bxbar, 0, 0.0, nz, c->s)));
+            TRY(kerr(launch_fill(c->bybar, 0, 0.0, nz, c->s)));
+            c->sc.bzbar = 0.0;
+        }
+        MHD_HIP_CHECK(hipStreamSynchronize(c->s));
+    }
     std::vector<double> keep;
     const double keep_bz = c->sc.bzbar;
     if (mx) {
@@ -820,6 +853,9 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->h32 = c->wf32;
     c->wsz = c->wf32 ? sizeof(float) : sizeof(double);
     c->p = *p;
+    // src/initial.F90:305-318: F rescaled for the dynamo and kinematic problems
+    if (p->opts & MHD_OPT_DYNAMO) c->p.f = 1.0;
+    if (p->opts & MHD_OPT_KINEMATIC) c->p.f = 0.0;
     c->g.nx = nx; c->g.ny = ny; c->g.nz = nz; c->g.nxy = nx * ny; c->g.n3 = c->n3;
     c->icutx = (nx / 2) / 3;
     c->icuty = (ny / 2) / 3;
@@ -1029,7 +1065,9 @@ int mhd_static(mhd_ctx_t* c)
     // src/initial.F90:25-86 on the host (integer RNG), then upload
     const int nz = c->nz, m = c->nxy;
     const long n3 = c->n3;
-    std::vector<double> t(n3), r(n3), zero(n3, 0.0), one(n3, 1.0), bz(n3, 1.0);
+    // src/initial.F90:63-78: B = x between perfect conductors, B = z otherwise
+    const bool pc = (c->p.opts & MHD_OPT_PERFECT) != 0;
+    std::vector<double> t(n3), r(n3), zero(n3, 0.0), bx(n3, pc ? 1.0 : 0.0), bz(n3, pc ? 0.0 : 1.0);
     for (int k = 0; k < nz; ++k) {
         const double z = (double)k * c->dz;
         const double t0 = 1.0 + z * c->p.theta;
@@ -1045,7 +1083,7 @@ int mhd_static(mhd_ctx_t* c)
             for (int q = 0; q < m; ++q) bz[(long)k * m + q] += 2.0 * (ran0(&bseed) - 0.5) * c->p.b_amp;
         }
     }
-    return mhd_set_state(c, r.data(), t.data(), zero.data(), zero.data(), zero.data(), zero.data(), zero.data(),
+    return mhd_set_state(c, r.data(), t.data(), zero.data(), zero.data(), zero.data(), bx.data(), zero.data(),
                          bz.data());
 }
 
@@ -1275,7 +1313,8 @@ int mhd_step(mhd_ctx_t* c, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* l
                 last->ubar = ga[0];
                 last->vbar = ga[1];
                 last->kinenergy = ga[2];
-                last->magenergy = mag ? c->p.f * ga[3] : ga[3];
+                // src/main.F90:567-571
+                last->magenergy = (c->p.opts & MHD_OPT_KINEMATIC) ? ga[3] : c->p.f * ga[3];
                 last->nusselt = nu;
                 last->divbmax = db[0];
                 last->divbmin = db[1];

@@ -469,12 +469,15 @@ int launch_ab3(const AB3Args& a, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// boundary conditions (defaults: impermeable stress-free, fixed T, vertical B)
-// src/dbydz.F90:3814-3900 (u, v, w), 4119 and 4183 (T), 4660-4750 (pol, tor,
-// means).  elf = 1/11.
+// boundary conditions: impermeable stress-free walls, src/dbydz.F90:3814-3900
+// (u, v, w); fixed T (4119, 4183) or the radiative top / fixed-flux bottom
+// options; vertical field (4660-4750) or perfect conductors (4314-4400) for
+// pol, tor and the means.  elf = 1/11.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NT) boundary_kernel(BCArgs a)
+__global__ void __launch_bounds__(NT) boundary_kernel(BCArgs av)
 {
+    (void)av;
+    const BCArgs& a = kernarg<BCArgs>();
     const long p = (long)blockIdx.x * NT + threadIdx.x;
     const long m = a.g.nxy;
     const double elf = 1.0 / 11.0;
@@ -483,11 +486,28 @@ __global__ void __launch_bounds__(NT) boundary_kernel(BCArgs a)
         a.w[p] = 0.0;
         a.u[p] = elf * (18.0 * a.u[m + p] - 9.0 * a.u[2 * m + p] + 2.0 * a.u[3 * m + p]);
         a.v[p] = elf * (18.0 * a.v[m + p] - 9.0 * a.v[2 * m + p] + 2.0 * a.v[3 * m + p]);
-        a.t[p] = 1.0;
+        if (a.radbc) {
+            // 4037-4057: one-sided dT/dz = theta T^4, ten Newton steps from the old value
+            const double tdz = a.theta * a.dz;
+            const double rest = -18.0 * a.t[m + p] + 9.0 * a.t[2 * m + p] - 2.0 * a.t[3 * m + p];
+            double t1 = a.t[p];
+            for (int it = 0; it < 10; ++it) {
+                const double ft1 = 6.0 * tdz * t1 * t1 * t1 * t1 + 11.0 * t1 + rest;
+                const double dft1 = 24.0 * tdz * t1 * t1 * t1 + 11.0;
+                t1 = t1 - (ft1 / dft1);
+            }
+            a.t[p] = t1;
+        } else {
+            a.t[p] = 1.0;                                  // 4119
+        }
         a.w[L + p] = 0.0;
         a.u[L + p] = elf * (18.0 * a.u[L - m + p] - 9.0 * a.u[L - 2 * m + p] + 2.0 * a.u[L - 3 * m + p]);
         a.v[L + p] = elf * (18.0 * a.v[L - m + p] - 9.0 * a.v[L - 2 * m + p] + 2.0 * a.v[L - 3 * m + p]);
-        a.t[L + p] = 1.0 + a.theta;
+        if (a.constflux)                                   // 4133-4143: dT/dz = theta
+            a.t[L + p] = elf * (18.0 * a.t[L - m + p] - 9.0 * a.t[L - 2 * m + p] + 2.0 * a.t[L - 3 * m + p] +
+                                6.0 * a.theta * a.dz);
+        else
+            a.t[L + p] = 1.0 + a.theta;                    // 4183
         if (a.su) {
             const long rows[2] = {p, L + p};
             const int ks[2] = {0, a.g.nz - 1};
@@ -501,7 +521,15 @@ __global__ void __launch_bounds__(NT) boundary_kernel(BCArgs a)
     }
     if (p < m && a.mag) {
         const long L = (long)(a.g.nz - 1) * m;
-        {
+        if (a.perfect) {
+            // 4314-4400: bz = 0 (pol), zero d/dz of bx, by (tor)
+            a.pol[p] = 0.0;
+            a.tor[p] = elf * (18.0 * a.tor[m + p] - 9.0 * a.tor[2 * m + p] + 2.0 * a.tor[3 * m + p]);
+            a.pol[L + p] = 0.0;
+            a.tor[L + p] =
+                elf * (18.0 * a.tor[L - m + p] - 9.0 * a.tor[L - 2 * m + p] + 2.0 * a.tor[L - 3 * m + p]);
+        } else {
+            // 4660-4750: vertical field
             a.tor[p] = 0.0;
             a.pol[p] = elf * (18.0 * a.pol[m + p] - 9.0 * a.pol[2 * m + p] + 2.0 * a.pol[3 * m + p]);
             a.tor[L + p] = 0.0;
@@ -510,10 +538,18 @@ __global__ void __launch_bounds__(NT) boundary_kernel(BCArgs a)
         }
     }
     if (p == 0 && a.mag) {
-        a.bxbar[0] = 0.0;
-        a.bybar[0] = 0.0;
-        a.bxbar[a.g.nz - 1] = 0.0;
-        a.bybar[a.g.nz - 1] = 0.0;
+        const int nz = a.g.nz;
+        if (a.perfect) {
+            a.bxbar[0] = elf * (18.0 * a.bxbar[1] - 9.0 * a.bxbar[2] + 2.0 * a.bxbar[3]);
+            a.bybar[0] = elf * (18.0 * a.bybar[1] - 9.0 * a.bybar[2] + 2.0 * a.bybar[3]);
+            a.bxbar[nz - 1] = elf * (18.0 * a.bxbar[nz - 2] - 9.0 * a.bxbar[nz - 3] + 2.0 * a.bxbar[nz - 4]);
+            a.bybar[nz - 1] = elf * (18.0 * a.bybar[nz - 2] - 9.0 * a.bybar[nz - 3] + 2.0 * a.bybar[nz - 4]);
+        } else {
+            a.bxbar[0] = 0.0;
+            a.bybar[0] = 0.0;
+            a.bxbar[nz - 1] = 0.0;
+            a.bybar[nz - 1] = 0.0;
+        }
     }
 }
 

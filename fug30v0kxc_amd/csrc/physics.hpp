@@ -122,6 +122,8 @@ struct BCArgs {
     const double* t0;          // mixed: background subtracted for st
     int hydro;   // boundary()
     int mag;     // mag_boundary()
+    int radbc, constflux, perfect;   // switches (MHD_OPT_*)
+    double dz;
 };
 int launch_boundary(const BCArgs& a, hipStream_t s);
 int launch_magdealias(Geom g, double* pol, double* tor, int icutx, int icuty, hipStream_t s);

@@ -9,12 +9,15 @@ module mhd_capi
   implicit none
 
   integer(c_int), parameter :: MHD_PREC_FP64 = 0, MHD_PREC_MIXED = 1
+  integer(c_int), parameter :: MHD_OPT_RADBC = 1, MHD_OPT_CONSTFLUX = 2, MHD_OPT_PERFECT = 4, &
+                               MHD_OPT_KINEMATIC = 8, MHD_OPT_DYNAMO = 16
   integer(c_int), parameter :: MHD_F_U = 0, MHD_F_V = 1, MHD_F_W = 2, MHD_F_T = 3, &
                                MHD_F_R = 4, MHD_F_BX = 5, MHD_F_BY = 6, MHD_F_BZ = 7
 
   type, bind(C) :: mhd_params_t
     real(c_double) :: gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax, sf
     real(c_double) :: t_amp, b_amp
+    integer(c_int) :: opts = 0
   end type
 
   type, bind(C) :: mhd_scalars_t

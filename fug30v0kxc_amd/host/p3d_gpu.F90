@@ -14,6 +14,8 @@
 !   t_amp     <real>   T noise amplitude of the static start (default 0.1)
 !   b_amp     <real>   Bz noise amplitude (default 0)
 !   precision fp64|mixed
+!   radbc | constflux | perfect | kinematic | dynamo   1 to switch on the
+!             reference's compile-time option (src/main.F90:109-116)
 !-----------------------------------------------------------------------
 program p3d_gpu
   use iso_c_binding
@@ -231,6 +233,16 @@ contains
         read(val, *) prm%b_amp
       case ('precision')
         if (trim(val) == 'mixed') prec = MHD_PREC_MIXED
+      case ('radbc')
+        if (trim(val) == '1') prm%opts = ior(prm%opts, MHD_OPT_RADBC)
+      case ('constflux')
+        if (trim(val) == '1') prm%opts = ior(prm%opts, MHD_OPT_CONSTFLUX)
+      case ('perfect')
+        if (trim(val) == '1') prm%opts = ior(prm%opts, MHD_OPT_PERFECT)
+      case ('kinematic')
+        if (trim(val) == '1') prm%opts = ior(prm%opts, MHD_OPT_KINEMATIC)
+      case ('dynamo')
+        if (trim(val) == '1') prm%opts = ior(prm%opts, MHD_OPT_DYNAMO)
       end select
     end do
     close(4)

@@ -50,7 +50,18 @@ typedef struct mhd_params {
     double gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax, sf;
     double t_amp;   /* T noise amplitude of static() (reference hard-codes 0.1) */
     double b_amp;   /* Bz noise amplitude (extension; 0 = reference) */
+    int opts;       /* MHD_OPT_* switches of src/control.f90:23-34 (0 = reference defaults) */
 } mhd_params_t;
+
+/* The reference's physics switches (src/main.F90:109-116, all off there):
+ * RADBC     radiative top T condition, dT/dz = theta T^4 (src/dbydz.F90:4037-4110)
+ * CONSTFLUX fixed-flux bottom T condition, dT/dz = theta (src/dbydz.F90:4133-4171)
+ * PERFECT   perfectly conducting walls (src/dbydz.F90:4227-4400), static B = x
+ * KINEMATIC F = 0 with the field still evolved (src/initial.F90:314-318)
+ * DYNAMO    F = 1 and a seed field at mag_init (src/initial.F90:305-309,
+ *           src/magnetic.f90:29-57) */
+enum { MHD_OPT_RADBC = 1, MHD_OPT_CONSTFLUX = 2, MHD_OPT_PERFECT = 4, MHD_OPT_KINEMATIC = 8,
+       MHD_OPT_DYNAMO = 16 };
 
 /* Driver scalars of src/main.F90 (dt history, AB start-up flags, counters). */
 typedef struct mhd_scalars {

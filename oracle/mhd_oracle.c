@@ -356,6 +356,9 @@ orc_ctx* orc_create(const orc_params* p, int nx, int ny, int nz, int* ier)
     c->nxy = (size_t)nx * ny;
     c->n3 = c->nxy * nz;
     c->p = *p;
+    /* src/initial.F90:305-318: F rescaled for the dynamo and kinematic problems */
+    if (c->p.opts & ORC_OPT_DYNAMO) c->p.f = 1.0;
+    if (c->p.opts & ORC_OPT_KINEMATIC) c->p.f = 0.0;
     for (int id = 0; id < ORC_NFIELDS; id++) {
         size_t len;
         if (id >= ORC_UDOT && id <= ORC_TORDOT) len = 3 * c->n3;
@@ -899,9 +902,11 @@ void orc_static(orc_ctx* c)
             w[k * m + q] = 0.0;
             r[k * m + q] = r0;
             t[k * m + q] = t[k * m + q] + t0;
-            bx[k * m + q] = 0.0;
+            /* src/initial.F90:63-78: horizontal field between perfect conductors */
+            const int pc = (c->p.opts & ORC_OPT_PERFECT) != 0;
+            bx[k * m + q] = pc ? 1.0 : 0.0;
             by[k * m + q] = 0.0;
-            bz[k * m + q] = 1.0;
+            bz[k * m + q] = pc ? 0.0 : 1.0;
         }
         if (c->p.b_amp != 0.0) {
             int iseed = 1996 + 64 * (k + 1) + 32;
@@ -1036,6 +1041,25 @@ int orc_mag_init(orc_ctx* c)
     double *ex = c->fld[ORC_EX], *ey = c->fld[ORC_EY], *ez = c->fld[ORC_EZ];
     double *wk3 = c->fld[ORC_WK3], *wk4 = c->fld[ORC_WK4];
     double *pol = c->fld[ORC_POL], *tor = c->fld[ORC_TOR];
+    if (c->p.opts & ORC_OPT_DYNAMO) {
+        /* src/magnetic.f90:29-57: seed field (1-based i, j in the cosines) */
+        const double seed = 0.003, tx = 8.0 * atan(1.0) / (double)nx, ty = 8.0 * atan(1.0) / (double)ny;
+        const int pc = (c->p.opts & ORC_OPT_PERFECT) != 0;
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    const size_t q = IDX(c, i, j, k);
+                    if (pc) {
+                        c->fld[ORC_BX][q] = seed * cos(ty * (double)(j + 1));
+                        c->fld[ORC_BY][q] = seed * cos(tx * (double)(i + 1));
+                        c->fld[ORC_BZ][q] = 0.0;
+                    } else {
+                        c->fld[ORC_BX][q] = 0.0;
+                        c->fld[ORC_BY][q] = 0.0;
+                        c->fld[ORC_BZ][q] = seed * cos(tx * (double)(i + 1)) * cos(ty * (double)(j + 1));
+                    }
+                }
+    }
     orc_forwardfft(c, c->fld[ORC_BX], ex);
     orc_forwardfft(c, c->fld[ORC_BY], ey);
     orc_forwardfft(c, c->fld[ORC_BZ], ez);
@@ -1419,13 +1443,36 @@ void orc_boundary(orc_ctx* c)
         u[p] = elf * (18.0 * u[m + p] - 9.0 * u[2 * m + p] + 2.0 * u[3 * m + p]);
         v[p] = elf * (18.0 * v[m + p] - 9.0 * v[2 * m + p] + 2.0 * v[3 * m + p]);
     }
-    for (size_t p = 0; p < m; p++) t[p] = 1.0;          /* 4119 */
+    if (c->p.opts & ORC_OPT_RADBC) {
+        /* 4037-4057: heat flux matched to a blackbody term, dT/dz = theta T^4
+         * by the one-sided third-order stencil; ten Newton steps */
+        const double th = c->p.theta, dz = c->dz;
+        for (size_t p = 0; p < m; p++) {
+            double t1 = t[p], ft1 = 0.0, dft1 = 1.0;
+            for (int it = 0; it < 10; it++) {
+                ft1 = 6.0 * th * dz * t1 * t1 * t1 * t1 + 11.0 * t1 - 18.0 * t[m + p] + 9.0 * t[2 * m + p]
+                      - 2.0 * t[3 * m + p];
+                dft1 = 24.0 * th * dz * t1 * t1 * t1 + 11.0;
+                t1 = t1 - (ft1 / dft1);
+            }
+            t[p] = t1;
+        }
+    } else {
+        for (size_t p = 0; p < m; p++) t[p] = 1.0;      /* 4119 */
+    }
     for (size_t p = 0; p < m; p++) w[L + p] = 0.0;
     for (size_t p = 0; p < m; p++) {
         u[L + p] = elf * (18.0 * u[L - m + p] - 9.0 * u[L - 2 * m + p] + 2.0 * u[L - 3 * m + p]);
         v[L + p] = elf * (18.0 * v[L - m + p] - 9.0 * v[L - 2 * m + p] + 2.0 * v[L - 3 * m + p]);
     }
-    for (size_t p = 0; p < m; p++) t[L + p] = 1.0 + c->p.theta; /* 4183 */
+    if (c->p.opts & ORC_OPT_CONSTFLUX) {
+        /* 4133-4143: dT/dz = theta at the bottom */
+        for (size_t p = 0; p < m; p++)
+            t[L + p] = elf * (18.0 * t[L - m + p] - 9.0 * t[L - 2 * m + p] + 2.0 * t[L - 3 * m + p]
+                              + 6.0 * c->p.theta * c->dz);
+    } else {
+        for (size_t p = 0; p < m; p++) t[L + p] = 1.0 + c->p.theta; /* 4183 */
+    }
 }
 
 void orc_mag_boundary(orc_ctx* c)
@@ -1434,6 +1481,21 @@ void orc_mag_boundary(orc_ctx* c)
     const int nz = c->nz;
     const size_t L = (size_t)(nz - 1) * m;
     double *pol = c->fld[ORC_POL], *tor = c->fld[ORC_TOR];
+    if (c->p.opts & ORC_OPT_PERFECT) {
+        /* 4314-4400: pol = 0 (bz = 0), zero d/dz of bx, by (tor and the means) */
+        double *bxb = c->fld[ORC_BXBAR], *byb = c->fld[ORC_BYBAR];
+        for (size_t p = 0; p < m; p++) pol[p] = 0.0;
+        bxb[0] = elf * (18.0 * bxb[1] - 9.0 * bxb[2] + 2.0 * bxb[3]);
+        byb[0] = elf * (18.0 * byb[1] - 9.0 * byb[2] + 2.0 * byb[3]);
+        for (size_t p = 0; p < m; p++)
+            tor[p] = elf * (18.0 * tor[m + p] - 9.0 * tor[2 * m + p] + 2.0 * tor[3 * m + p]);
+        for (size_t p = 0; p < m; p++) pol[L + p] = 0.0;
+        bxb[nz - 1] = elf * (18.0 * bxb[nz - 2] - 9.0 * bxb[nz - 3] + 2.0 * bxb[nz - 4]);
+        byb[nz - 1] = elf * (18.0 * byb[nz - 2] - 9.0 * byb[nz - 3] + 2.0 * byb[nz - 4]);
+        for (size_t p = 0; p < m; p++)
+            tor[L + p] = elf * (18.0 * tor[L - m + p] - 9.0 * tor[L - 2 * m + p] + 2.0 * tor[L - 3 * m + p]);
+        return;
+    }
     c->fld[ORC_BXBAR][0] = 0.0;
     c->fld[ORC_BYBAR][0] = 0.0;
     for (size_t p = 0; p < m; p++) tor[p] = 0.0;
@@ -1604,7 +1666,8 @@ int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag)
 {
     const size_t n3 = c->n3;
     const int nz = c->nz;
-    const int mag = (c->p.f != 0.0);
+    /* src/main.F90:355-375: the field evolves when F != 0 or for the kinematic problem */
+    const int mag = (c->p.f != 0.0) || (c->p.opts & ORC_OPT_KINEMATIC);
     double* dots[7] = { c->fld[ORC_UDOT], c->fld[ORC_VDOT], c->fld[ORC_WDOT], c->fld[ORC_RDOT],
                         c->fld[ORC_TDOT], c->fld[ORC_POLDOT], c->fld[ORC_TORDOT] };
     double* vars[7] = { c->fld[ORC_U], c->fld[ORC_V], c->fld[ORC_W], c->fld[ORC_R],
@@ -1684,7 +1747,8 @@ int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag)
             for (int id = ORC_WK1; id <= ORC_WK3; id++) memset(c->fld[id], 0, sizeof(double) * n3);
             if (diag) {
                 diag[0] = ga[0]; diag[1] = ga[1]; diag[2] = ga[2];
-                diag[3] = mag ? c->p.f * ga[3] : ga[3];
+                /* src/main.F90:567-571 */
+                diag[3] = (c->p.opts & ORC_OPT_KINEMATIC) ? ga[3] : c->p.f * ga[3];
                 diag[4] = nu; diag[5] = db[0]; diag[6] = db[1];
             }
             double divbtest = fmax(fabs(db[0]), fabs(db[1]));

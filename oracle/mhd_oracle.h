@@ -33,7 +33,17 @@ typedef struct orc_params {
     /* extensions (defaults reproduce the reference):                       */
     double t_amp;      /* T perturbation amplitude (reference: 0.1)          */
     double b_amp;      /* Bz perturbation amplitude (reference: 0, absent)   */
+    int opts;          /* ORC_OPT_* switches of src/control.f90:23-34 (all off) */
 } orc_params;
+
+/* the reference's compile-time switches (src/main.F90:109-116) */
+enum {
+    ORC_OPT_RADBC = 1,      /* radiative top T condition, src/dbydz.F90:4037-4110 */
+    ORC_OPT_CONSTFLUX = 2,  /* fixed-flux bottom T condition, 4133-4171          */
+    ORC_OPT_PERFECT = 4,    /* perfectly conducting walls, 4227-4400; B0 = x     */
+    ORC_OPT_KINEMATIC = 8,  /* F = 0 but B evolved, src/initial.F90:314-318       */
+    ORC_OPT_DYNAMO = 16     /* F = 1 and a seed field, src/magnetic.f90:29-57    */
+};
 
 /* field ids for orc_field() */
 enum {

@@ -25,10 +25,14 @@ SCALARS = ["dt", "dt1", "dt2", "ttime", "bzbar", "nit", "iter", "first", "second
 SID = {n: i for i, n in enumerate(SCALARS)}
 
 
+# switches of orc_params.opts (src/control.f90:23-34)
+OPT_RADBC, OPT_CONSTFLUX, OPT_PERFECT, OPT_KINEMATIC, OPT_DYNAMO = 1, 2, 4, 8, 16
+
+
 class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("gamma", "sigma", "theta", "ck", "cm", "f", "tayl", "psi", "zeta",
-                 "xmax", "ymax", "sf", "t_amp", "b_amp")]
+                 "xmax", "ymax", "sf", "t_amp", "b_amp")] + [("opts", ctypes.c_int)]
 
 
 _lib = None
@@ -377,5 +381,8 @@ class Oracle:
         """static() + ini_trans_params() + mag_init(), src/main.F90:275-333."""
         self.static()
         self.ini_trans_params()
-        if self.params["f"] != 0.0:
+        # src/main.F90:320-333 (F as rescaled for the dynamo / kinematic problems)
+        opts = self.params.get("opts", 0)
+        f = 0.0 if opts & OPT_KINEMATIC else (1.0 if opts & OPT_DYNAMO else self.params["f"])
+        if f != 0.0 or opts & OPT_KINEMATIC:
             self.mag_init()
