@@ -44,9 +44,10 @@ KERNELS = {"none": 0, "xderiv2": 1, "yderiv2": 2, "zderiv": 3, "hydro": 4, "ab3"
 
 
 class MhdParams(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_double) for n in
+    _fields_ = ([(n, ctypes.c_double) for n in
                 ("gamma", "sigma", "theta", "ck", "cm", "f", "tayl", "psi", "zeta",
-                 "xmax", "ymax", "sf", "t_amp", "b_amp")] + [("opts", ctypes.c_int)]
+                 "xmax", "ymax", "sf", "t_amp", "b_amp", "eta_c", "eta_z0", "eta_w")]
+                + [("opts", ctypes.c_int)])
 
 
 class MhdScalars(ctypes.Structure):

@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(HNT) hydro_col_kernel(HydroColArgs av)
         const double exv = -tt * rx, eyv = -tt * ry;
         double tdot = -rr * (u0 * tx + v0 * ty + w0 * d.tzu) +
                       p.gck * kap * (at(a.txx, q) + at(a.tyy, q) + d.tzz) + p.gck * dkap * d.tzu +
-                      p.fzckg * at(a.jsq, q);
+                      p.fzckg * (a.etar ? a.etar[k] : 1.0) * at(a.jsq, q);
         double udot = exv - rr * tx;
         double vdot = eyv - rr * ty;
         double wdot = -rr * d.tzc;

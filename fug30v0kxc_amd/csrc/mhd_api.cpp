@@ -68,6 +68,9 @@ struct mhd_ctx {
     double* mhist[2][3];
     float *su, *sv, *sw, *st, *sr;   // mixed: FP32 fluctuation shadows of u v w t r
     double *t0, *r0, *bzm;           // mixed: backgrounds of t, r; Bz-bar per plane
+    int eta_on;                      // eta(z) extension active
+    double *etar, *eta1;             // eta(z)/eta_0 and zeta C_k (etar - 1), nz entries
+    void *jx, *jy, *jz;              // eta(z): current density kept for the EMF
     int ring[3];
     void* wk[W_COUNT];
     double *red_partial, *red_result, *d_bzbar;
@@ -324,6 +327,7 @@ HydroColArgs colargs(mhd_ctx* c)
     }
     a.bx = c->bx; a.by = c->by; a.bz = c->bz; a.jsq = c->jsq;
     a.kappa = c->kappa; a.dkappa = c->dkappa;
+    a.etar = c->eta_on ? c->etar : nullptr;
     void** W = c->wk;
     a.rx = W[W_RX]; a.tx = W[W_TX]; a.txx = W[W_TXX]; a.ux = W[W_UX]; a.uxxc = W[W_UXX];
     a.vx = W[W_VX]; a.vxx = W[W_VXX]; a.wx = W[W_WX]; a.wxx = W[W_WXX]; a.xbb = W[W_XBB]; a.xzx = W[W_XZX];
@@ -362,6 +366,7 @@ MagArgs margs(mhd_ctx* c)
     a.bxh = W[W_WX]; a.byh = W[W_WXX]; a.bzh = W[W_XBB]; a.jzh = W[W_XZX]; a.dybzh = W[W_RY]; a.dxbzh = W[W_TY];
     a.bx = c->bx; a.by = c->by; a.jsq = c->jsq;
     if (c->su) { a.bxm = c->bxbar; a.bym = c->bybar; }
+    if (c->eta_on) { a.jx = c->jx; a.jy = c->jy; a.jz = c->jz; }
     a.wf32 = c->wf32;
     return a;
 }
@@ -481,6 +486,12 @@ int run_mag_deriv(mhd_ctx* c)
         L.job[2] = sjob(U_, wf, m.ezh, nullptr, 0, 0, wf);
         L.job[2].load_op = LD_CROSS; L.job[2].in[1] = c->by; L.job[2].in[2] = V_; L.job[2].in[3] = c->bx;
         L.job[2].mean[1] = mby; L.job[2].mean[3] = mbx;
+        if (c->eta_on) {
+            // eta(z) extension: E = u x B - zeta C_k (eta/eta_0 - 1) J
+            L.job[0].jin = c->jx; L.job[0].jcoef = c->eta1;
+            L.job[1].jin = c->jy; L.job[1].jcoef = c->eta1;
+            L.job[2].jin = c->jz; L.job[2].jcoef = c->eta1;
+        }
         L.njobs = 3;
         TRY(spec(c, L));
         SpecLaunch M = spec_base(c, 1, SM_R2HC);
@@ -889,6 +900,8 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     bytes += 4 * fw;                                  // bx by bz jsq
     bytes += 21 * fh + 6 * vz;                        // history
     bytes += 5 * fs + 3 * vz;                         // shadows, backgrounds
+    c->eta_on = p->eta_c > 1.0;
+    bytes += 2 * vz + (c->eta_on ? 3 * fw : 0);       // eta(z) profile, J
     bytes += W_COUNT * fw;                            // work
     bytes += al256(nz * 8 * sizeof(double)) + 2 * al256(8 * sizeof(double));  // reductions
     if (hipMalloc((void**)&c->pool, bytes) != hipSuccess) {
@@ -934,6 +947,11 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->t0 = (double*)take(vz);
     c->r0 = (double*)take(vz);
     c->bzm = (double*)take(vz);
+    c->etar = (double*)take(vz);
+    c->eta1 = (double*)take(vz);
+    if (c->eta_on) {
+        c->jx = take(fw); c->jy = take(fw); c->jz = take(fw);
+    }
     if (c->wf32) {
         c->su = (float*)take(fs); c->sv = (float*)take(fs); c->sw = (float*)take(fs);
         c->st = (float*)take(fs); c->sr = (float*)take(fs);
@@ -971,6 +989,16 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
         }
         e |= hipMemcpy(c->t0, t0.data(), nz * sizeof(double), hipMemcpyHostToDevice);
         e |= hipMemcpy(c->r0, r0.data(), nz * sizeof(double), hipMemcpyHostToDevice);
+        // eta(z)/eta_0 = 1 + (eta_c - 1)(1 + tanh((z - eta_z0)/eta_w))/2 (extension)
+        std::vector<double> er(nz, 1.0), e1(nz, 0.0);
+        if (c->eta_on)
+            for (int k = 0; k < nz; ++k) {
+                const double z = (double)k * c->dz;
+                er[k] = 1.0 + (p->eta_c - 1.0) * 0.5 * (1.0 + std::tanh((z - p->eta_z0) / p->eta_w));
+                e1[k] = c->ph.zck * (er[k] - 1.0);
+            }
+        e |= hipMemcpy(c->etar, er.data(), nz * sizeof(double), hipMemcpyHostToDevice);
+        e |= hipMemcpy(c->eta1, e1.data(), nz * sizeof(double), hipMemcpyHostToDevice);
     }
     twiddles(nx, tw64, tw32);
     e |= hipMemcpy(twx64, tw64.data(), 2 * nx * sizeof(double), hipMemcpyHostToDevice);

@@ -300,6 +300,11 @@ __global__ void __launch_bounds__(NT) jsq_kernel(MagArgs a)
     const double ey = (double)((const TW*)a.dxbzh)[q];
     const double ez = (double)((const TW*)a.jzh)[q];
     ((TW*)a.jsq)[q] = (TW)(ez * ez + ((ex - wk1) * (ex - wk1)) + ((wk2 - ey) * (wk2 - ey)));
+    if (a.jx) {
+        ((TW*)a.jx)[q] = (TW)(ex - wk1);
+        ((TW*)a.jy)[q] = (TW)(wk2 - ey);
+        ((TW*)a.jz)[q] = (TW)ez;
+    }
 }
 
 // mag_init spectral part, src/magnetic.f90:63-233 (truncation folded into reads)

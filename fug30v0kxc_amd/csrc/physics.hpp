@@ -57,6 +57,7 @@ struct HydroColArgs {
     const double *bxm, *bym;            // mixed: Bx-bar(z), By-bar(z)
     double bzm;                         // mixed: Bz-bar
     const double *kappa, *dkappa;
+    const double* etar;                 // eta(z)/eta_0 for the Ohmic heating, or null (1)
     // x pass: d/dx of r, t, u, v, w; d2/dx2 of t, v, w; uxxc = sck*uxx +
     // scko3*d/dx(d/dx u); xbb = d/dx(F by bx); xzx = d/dx(F bz bx);
     // dxm = d/dx(F/2 (bx^2 - by^2 - bz^2))
@@ -85,6 +86,7 @@ struct MagArgs {
     const void *bx, *by;          // physical B for J^2
     const double *bxm, *bym;      // mixed: means of bx, by (their d/dz enters J^2)
     void *jsq;
+    void *jx, *jy, *jz;           // eta(z): J kept for the EMF, or null
     int wf32;
 };
 int launch_mag_spec1(const MagArgs& a, hipStream_t s);   // D, C from E-hat

@@ -32,6 +32,10 @@ struct SpecJob {
     int load_op;      // LoadOp
     double scale;
     const double* mean[4]; // optional per-plane means of in[0..3] (nz entries)
+    // LD_CROSS only: the load also subtracts jcoef[k] * jin (the eta(z) term
+    // -(eta - eta_0) J of the EMF), when jin is set
+    const void* jin;
+    const double* jcoef;
     double mc[2];     // MUL_D2C coefficients
     const double* bg; // optional per-plane value subtracted on load (nz entries)
     void* out[2];

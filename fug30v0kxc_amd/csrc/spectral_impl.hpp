@@ -59,6 +59,8 @@ struct Loader {
     TC scale;
     double bgk;
     TC m0, m1, m2, m3;   // per-plane means of the factors (0 when absent)
+    const TC* __restrict__ jj;   // LD_CROSS: current density component, or null
+    TC jc;                       // its per-plane coefficient
     __device__ __forceinline__ Loader(const SpecJob& J, int k)
     {
         a = (const TC*)J.in[0];
@@ -69,6 +71,14 @@ struct Loader {
         scale = (TC)J.scale;
         bgk = (LOP == LD_BG && J.bg) ? J.bg[k] : 0.0;
         m0 = m1 = m2 = m3 = (TC)0;
+        jj = nullptr;
+        jc = (TC)0;
+        if constexpr (LOP == LD_CROSS) {
+            if (J.jin) {
+                jj = (const TC*)J.jin;
+                jc = (TC)J.jcoef[k];
+            }
+        }
         if constexpr (LOP == LD_PROD || LOP == LD_CROSS || LOP == LD_MAXW) {
             if (J.mean[0]) m0 = (TC)J.mean[0][k];
             if (J.mean[1]) m1 = (TC)J.mean[1][k];
@@ -95,8 +105,14 @@ struct Loader {
                         (TC)2 * scale * (m0 * x.y - m1 * y.y - m2 * z.y)};
         } else {
             const cpx<TC> x = ld_pair(a, i), y = ld_pair(b, i), z = ld_pair(c, i), w = ld_pair(d, i);
-            return {x.x * y.x - z.x * w.x + (x.x * m1 + m0 * y.x) - (z.x * m3 + m2 * w.x),
-                    x.y * y.y - z.y * w.y + (x.y * m1 + m0 * y.y) - (z.y * m3 + m2 * w.y)};
+            cpx<TC> r = {x.x * y.x - z.x * w.x + (x.x * m1 + m0 * y.x) - (z.x * m3 + m2 * w.x),
+                         x.y * y.y - z.y * w.y + (x.y * m1 + m0 * y.y) - (z.y * m3 + m2 * w.y)};
+            if (jj) {
+                const cpx<TC> q = ld_pair(jj, i);
+                r.x = r.x - jc * q.x;
+                r.y = r.y - jc * q.y;
+            }
+            return r;
         }
     }
     __device__ __forceinline__ TC operator()(long i) const
@@ -113,7 +129,9 @@ struct Loader {
                    (TC)2 * scale * (m0 * x - m1 * y - m2 * z);
         } else {
             const TC x = a[i], y = b[i], z = c[i], w = d[i];
-            return x * y - z * w + (x * m1 + m0 * y) - (z * m3 + m2 * w);
+            TC r = x * y - z * w + (x * m1 + m0 * y) - (z * m3 + m2 * w);
+            if (jj) r = r - jc * jj[i];
+            return r;
         }
     }
 };

@@ -17,6 +17,7 @@ module mhd_capi
   type, bind(C) :: mhd_params_t
     real(c_double) :: gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax, sf
     real(c_double) :: t_amp, b_amp
+    real(c_double) :: eta_c = 0.0d0, eta_z0 = 0.5d0, eta_w = 0.05d0
     integer(c_int) :: opts = 0
   end type
 

@@ -14,6 +14,8 @@
 !   t_amp     <real>   T noise amplitude of the static start (default 0.1)
 !   b_amp     <real>   Bz noise amplitude (default 0)
 !   precision fp64|mixed
+!   eta_c, eta_z0, eta_w  <real>  eta(z) step: contrast, centre, width
+!             (extension; contrast <= 1 keeps the reference's constant eta)
 !   radbc | constflux | perfect | kinematic | dynamo   1 to switch on the
 !             reference's compile-time option (src/main.F90:109-116)
 !-----------------------------------------------------------------------
@@ -233,6 +235,12 @@ contains
         read(val, *) prm%b_amp
       case ('precision')
         if (trim(val) == 'mixed') prec = MHD_PREC_MIXED
+      case ('eta_c')
+        read(val, *) prm%eta_c
+      case ('eta_z0')
+        read(val, *) prm%eta_z0
+      case ('eta_w')
+        read(val, *) prm%eta_w
       case ('radbc')
         if (trim(val) == '1') prm%opts = ior(prm%opts, MHD_OPT_RADBC)
       case ('constflux')

@@ -50,6 +50,9 @@ typedef struct mhd_params {
     double gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax, sf;
     double t_amp;   /* T noise amplitude of static() (reference hard-codes 0.1) */
     double b_amp;   /* Bz noise amplitude (extension; 0 = reference) */
+    /* magnetic diffusivity eta(z) = zeta C_k [1 + (eta_c - 1)(1 + tanh((z - eta_z0)
+     * / eta_w)) / 2] (extension; eta_c <= 1 keeps the reference's constant eta) */
+    double eta_c, eta_z0, eta_w;
     int opts;       /* MHD_OPT_* switches of src/control.f90:23-34 (0 = reference defaults) */
 } mhd_params_t;
 

@@ -366,6 +366,7 @@ orc_ctx* orc_create(const orc_params* p, int nx, int ny, int nz, int* ier)
         else if (id >= ORC_BXBAR && id <= ORC_DKAPPA) len = (size_t)nz;
         else if (id == ORC_AKX1 || id == ORC_AKX2) len = (size_t)nx;
         else if (id == ORC_AKY1 || id == ORC_AKY2) len = (size_t)ny;
+        else if (id == ORC_ETAR) len = (size_t)nz;
         else len = c->n3;
         c->fld[id] = (double*)calloc(len, sizeof(double));
         if (!c->fld[id]) { *ier = -5; orc_destroy(c); return NULL; }
@@ -378,6 +379,8 @@ orc_ctx* orc_create(const orc_params* p, int nx, int ny, int nz, int* ier)
     set_constants(c);
     init_fft(c);
     c->first = 1; c->second = 0;
+    for (int k = 0; k < nz; k++) c->fld[ORC_ETAR][k] = 1.0;
+    if (c->p.eta_c > 1.0) orc_ini_trans_params(c);
     return c;
 }
 
@@ -927,6 +930,14 @@ void orc_ini_trans_params(orc_ctx* c)
         ka[k] = 5.0e-1 * (tanh(constB) + 1);
         dka[k] = 5.0e-1 * constA * pow(cosh(constB), -2.0);
     }
+    /* eta(z) / eta_0 (1 everywhere for the reference's constant eta) */
+    double* er = c->fld[ORC_ETAR];
+    for (int k = 0; k < c->nz; k++) {
+        const double z = (double)k * c->dz;
+        er[k] = (c->p.eta_c > 1.0)
+                    ? 1.0 + (c->p.eta_c - 1.0) * 0.5 * (1.0 + tanh((z - c->p.eta_z0) / c->p.eta_w))
+                    : 1.0;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1007,6 +1018,9 @@ static void spec_invlap(orc_ctx* c, double* a)
 
 /* J^2 from (spectral) ez and physical bx,by; shared tail of mag_init and
  * bandj, src/magnetic.f90:199-261 and 584-645 */
+/* eta(z) extension: on when eta_c > 1 */
+static int eta_on(const orc_ctx* c) { return c->p.eta_c > 1.0; }
+
 static int jsq_tail(orc_ctx* c)
 {
     const size_t n3 = c->n3;
@@ -1026,6 +1040,16 @@ static int jsq_tail(orc_ctx* c)
     for (size_t q = 0; q < n3; q++)
         jsq[q] = ez[q] * ez[q] + ((ex[q] - wk1[q]) * (ex[q] - wk1[q]))
                  + ((wk2[q] - ey[q]) * (wk2[q] - ey[q]));
+    if (eta_on(c)) {
+        /* J = (dBz/dy - dBy/dz, dBx/dz - dBz/dx, Jz) for the eta(z) EMF */
+        double *jx = c->fld[ORC_JX], *jy = c->fld[ORC_JY], *jz = c->fld[ORC_JZ];
+        ORC_PAR
+        for (size_t q = 0; q < n3; q++) {
+            jx[q] = ex[q] - wk1[q];
+            jy[q] = wk2[q] - ey[q];
+            jz[q] = ez[q];
+        }
+    }
     return 0;
 }
 
@@ -1132,6 +1156,23 @@ int orc_mag_deriv(orc_ctx* c)
         wk1[q] = v[q] * bz[q] - w[q] * by[q];
         wk2[q] = w[q] * bx[q] - u[q] * bz[q];
         wk3[q] = u[q] * by[q] - v[q] * bx[q];
+    }
+    if (eta_on(c)) {
+        /* eta(z) extension: E = u x B - (eta(z) - zeta C_k) J, so that curl E
+         * adds -curl((eta - eta_0) J) to the constant-eta induction terms */
+        const double* er = c->fld[ORC_ETAR];
+        const double *jx = c->fld[ORC_JX], *jy = c->fld[ORC_JY], *jz = c->fld[ORC_JZ];
+        const size_t m = c->nxy;
+        ORC_PAR
+        for (int k = 0; k < nz; k++) {
+            const double e1 = c->zck * (er[k] - 1.0);
+            for (size_t p = 0; p < m; p++) {
+                const size_t q = (size_t)k * m + p;
+                wk1[q] = wk1[q] - e1 * jx[q];
+                wk2[q] = wk2[q] - e1 * jy[q];
+                wk3[q] = wk3[q] - e1 * jz[q];
+            }
+        }
     }
     orc_forwardfft(c, wk1, ex);
     orc_forwardfft(c, wk2, ey);
@@ -1269,7 +1310,7 @@ int orc_hydro_deriv(orc_ctx* c)
             tdot[q] = -r[q] * (u0 * udot[q] + v0 * vdot[q] + w0 * wdot[q])
                       + c->gck * kap * (wk1[q] + wk2[q] + wk3[q])
                       + c->gck * dkap * wdot[q]
-                      + c->fzckg * jsq[q];
+                      + c->fzckg * c->fld[ORC_ETAR][k] * jsq[q];
             udot[q] = ex[q] - r[q] * udot[q];
             vdot[q] = ey[q] - r[q] * vdot[q];
             wdot[q] = -r[q] * wk4[q];

@@ -33,6 +33,9 @@ typedef struct orc_params {
     /* extensions (defaults reproduce the reference):                       */
     double t_amp;      /* T perturbation amplitude (reference: 0.1)          */
     double b_amp;      /* Bz perturbation amplitude (reference: 0, absent)   */
+    /* magnetic diffusivity eta(z) = zeta C_k [1 + (eta_c - 1)(1 + tanh((z -
+     * eta_z0) / eta_w)) / 2]; eta_c <= 1 keeps the reference's constant eta */
+    double eta_c, eta_z0, eta_w;
     int opts;          /* ORC_OPT_* switches of src/control.f90:23-34 (all off) */
 } orc_params;
 
@@ -53,6 +56,8 @@ enum {
     ORC_UDOT, ORC_VDOT, ORC_WDOT, ORC_TDOT, ORC_RDOT, ORC_POLDOT, ORC_TORDOT,
     ORC_BXBAR, ORC_BYBAR, ORC_KAPPA, ORC_DKAPPA, ORC_BXBARDOT, ORC_BYBARDOT,
     ORC_AKX1, ORC_AKX2, ORC_AKY1, ORC_AKY2,
+    ORC_JX, ORC_JY, ORC_JZ,   /* current density, kept when eta(z) is on */
+    ORC_ETAR,                 /* eta(z) / (zeta C_k), nz entries           */
     ORC_NFIELDS
 };
 

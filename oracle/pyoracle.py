@@ -18,7 +18,7 @@ _LIB = os.path.join(_HERE, "liboracle.so")
 FIELDS = ["u", "v", "w", "t", "r", "bx", "by", "bz", "pol", "tor", "jsq", "ex", "ey", "ez",
           "wk1", "wk2", "wk3", "wk4", "udot", "vdot", "wdot", "tdot", "rdot", "poldot",
           "tordot", "bxbar", "bybar", "kappa", "dkappa", "bxbardot", "bybardot",
-          "akx1", "akx2", "aky1", "aky2"]
+          "akx1", "akx2", "aky1", "aky2", "jx", "jy", "jz", "etar"]
 FID = {n: i for i, n in enumerate(FIELDS)}
 SCALARS = ["dt", "dt1", "dt2", "ttime", "bzbar", "nit", "iter", "first", "second",
            "gck", "sck", "zck", "dz", "xn", "xzn", "nusstmp"]
@@ -30,9 +30,10 @@ OPT_RADBC, OPT_CONSTFLUX, OPT_PERFECT, OPT_KINEMATIC, OPT_DYNAMO = 1, 2, 4, 8, 1
 
 
 class OrcParams(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_double) for n in
+    _fields_ = ([(n, ctypes.c_double) for n in
                 ("gamma", "sigma", "theta", "ck", "cm", "f", "tayl", "psi", "zeta",
-                 "xmax", "ymax", "sf", "t_amp", "b_amp")] + [("opts", ctypes.c_int)]
+                 "xmax", "ymax", "sf", "t_amp", "b_amp", "eta_c", "eta_z0", "eta_w")]
+                + [("opts", ctypes.c_int)])
 
 
 _lib = None
@@ -217,7 +218,7 @@ class Oracle:
             shape = (3, nz, ny, nx)
         elif name in ("bxbardot", "bybardot"):
             shape = (3, nz)
-        elif name in ("bxbar", "bybar", "kappa", "dkappa"):
+        elif name in ("bxbar", "bybar", "kappa", "dkappa", "etar"):
             shape = (nz,)
         elif name in ("akx1", "akx2"):
             shape = (nx,)

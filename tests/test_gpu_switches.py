@@ -71,3 +71,30 @@ def test_switch_steps_mixed():
     for key, i in (("kinenergy", 2), ("magenergy", 3), ("nusselt", 4)):
         assert abs(ds[key] - do[i]) <= 1e-5 * abs(do[i]), key
     s.close()
+
+
+ETA_CASES = {
+    "eta10@0.5": dict(eta_c=10.0, eta_z0=0.5, eta_w=0.05),
+    "eta100@0.3": dict(eta_c=100.0, eta_z0=0.3, eta_w=0.05),
+}
+
+
+@pytest.mark.parametrize("name", list(ETA_CASES))
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_eta_profile_steps(name, prec):
+    """eta(z) extension (E = u x B - (eta - eta_0) J, Ohmic heating eta J^2)."""
+    from fug30v0kxc_amd import MHD_PREC_FP64, MHD_PREC_MIXED, Solver
+    print(f"[switches] {name} {prec}", flush=True)
+    p = dict(reference_params(t_amp=1e-2, b_amp=1e-3), **ETA_CASES[name])
+    s = Solver(16, 16, 12, p, precision=MHD_PREC_FP64 if prec == "fp64" else MHD_PREC_MIXED)
+    s.static()
+    o = Oracle(16, 16, 12, p)
+    o.setup_static()
+    ds = s.step(20, 1e-4, 10)
+    do = o.step(20, 1e-4, 10)
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    for n, e in field_errors(s.get_state(), o).items():
+        assert e < tol, (name, n, e)
+    for key, i in (("kinenergy", 2), ("magenergy", 3), ("nusselt", 4)):
+        assert abs(ds[key] - do[i]) <= tol * abs(do[i]), (name, key)
+    s.close()
