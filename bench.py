@@ -33,13 +33,26 @@ sys.path.insert(0, ROOT)
 NX, NY, NZ = 256, 256, 160
 DT = 5e-5
 NDU06 = 100
+# the mixed-precision error is measured after this many steps of both modes
+ERR_STEPS = 2000
+
+# (nx, ny, nz, dt, extra parameters, description)
+WORKLOADS = {
+    "c2": (256, 256, 160, 5e-5, {},
+           "256x256x160 compressible MHD, Ra=1e6, Q=1000, T/Bz noise 1e-3, fixed dt 5e-5"),
+    "c3": (512, 512, 256, 5e-6, {"eta_c": 100.0, "eta_z0": 0.3, "eta_w": 0.05},
+           "512x512x256 compressible MHD, Ra=1e6, Q=1000, eta(z) contrast 100 at z=0.3 (width 0.05), "
+           "T/Bz noise 1e-3, fixed dt 5e-6"),
+}
 HBM_PEAK_GBS = 8000.0
 METRIC = "Mcell-steps/s at 256²×160: FP64 vs mixed-FP32; rel err (Nu, E) vs FP64 oracle"
 
 
-def workload_params():
+def workload_params(extra=None):
     from fug30v0kxc_amd import reference_params
-    return reference_params(ra=1e6, q=1000.0, t_amp=1e-3, b_amp=1e-3)
+    p = reference_params(ra=1e6, q=1000.0, t_amp=1e-3, b_amp=1e-3)
+    p.update(extra or {})
+    return p
 
 
 def cpu_baseline(params, steps=3, steps1=1):
@@ -69,27 +82,36 @@ def cpu_baseline(params, steps=3, steps1=1):
                       f"({dt1:.1f} s), each after 1 untimed step"}
 
 
-def run(precision, steps, warmup, kernel):
-    """Static start, `warmup` untimed steps, `steps` timed steps."""
+def run(precision, steps, warmup, kernel, wl):
+    """Static start, `warmup` untimed steps, `steps` timed steps; returns the
+    live solver with the timings."""
     from fug30v0kxc_amd import MHD_PREC_FP64, MHD_PREC_MIXED, Solver
+    nx, ny, nz, dt, extra, _ = wl
     prec = MHD_PREC_FP64 if precision == "fp64" else MHD_PREC_MIXED
-    s = Solver(NX, NY, NZ, workload_params(), precision=prec)
+    s = Solver(nx, ny, nz, workload_params(extra), precision=prec)
     s.static()
-    s.step(warmup, DT, NDU06)
+    s.step(warmup, dt, NDU06)
     s.sync()
     s.kernel_timing(kernel, True)
     s.sync()
     t0 = time.perf_counter()
-    diag = s.step(steps, DT, NDU06)
+    diag = s.step(steps, dt, NDU06)
     s.sync()
     t1 = time.perf_counter()
     kms, klaunch = s.kernel_time()
     s.kernel_timing(kernel, False)
-    st = s.get_state()
-    dev_bytes = s.device_bytes()
-    s.close()
-    return {"elapsed": t1 - t0, "diag": diag, "state": st, "kms": kms, "klaunch": klaunch,
-            "device_bytes": dev_bytes}
+    return {"solver": s, "elapsed": t1 - t0, "diag": diag, "kms": kms, "klaunch": klaunch,
+            "cells": nx * ny * nz, "device_bytes": s.device_bytes(), "done": warmup + steps}
+
+
+def advance(r, total, dt):
+    """Step a run on to `total` steps; returns the last diagnostics."""
+    left = total - r["done"]
+    d = r["diag"]
+    if left > 0:
+        d = r["solver"].step(left, dt, NDU06)
+        r["done"] = total
+    return d
 
 
 def measured_traffic(precision, kernel):
@@ -105,15 +127,16 @@ def measured_traffic(precision, kernel):
     return None
 
 
-def roofline_of(r, kernel, elem):
-    cells = NX * NY * NZ
+def roofline_of(r, kernel, elem, workload="c2"):
+    cells = r["cells"]
     # algorithmic bytes of one launch: 4 fields, 1 read + 2 writes each
     bytes_per_launch = 4 * 3 * elem * cells
     avg_ms = r["kms"] / max(r["klaunch"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = measured_traffic("fp64" if elem == 8 else "mixed", kernel) if workload == "c2" else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("fp64" if elem == 8 else "mixed", kernel),
+            "traffic": traffic,
             "kernel": f"spec_kernel {kernel} (4 fields: d/dx and d2/dx2 of T,u,v,w)",
             "avg_launch_ms": round(avg_ms, 4), "launches": r["klaunch"],
             "bytes_per_launch": bytes_per_launch}
@@ -130,6 +153,10 @@ def main():
                          "(e.g. for a kernel profile of one mode)")
     ap.add_argument("--kernel", default="xderiv2", choices=("xderiv2", "yderiv2"),
                     help="kernel family for the roofline object")
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS),
+                    help="c2: 256x256x160 (default); c3: 512x512x256 with the eta(z) step")
+    ap.add_argument("--err-steps", type=int, default=ERR_STEPS,
+                    help="steps after which the mixed run is compared with the FP64 run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -149,59 +176,69 @@ def main():
         return 2
 
     import numpy as np
-    cells = NX * NY * NZ
+    wl = WORKLOADS[args.workload]
+    nx, ny, nz, dt, extra, desc = wl
+    cells = nx * ny * nz
     if args.precision == "mixed":
-        rm = run("mixed", args.steps, args.warmup, args.kernel)
+        rm = run("mixed", args.steps, args.warmup, args.kernel, wl)
         vm = cells * args.steps / rm["elapsed"] / 1e6
         out = {"metric": METRIC, "value": round(vm, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
                "steps": args.steps, "warmup": args.warmup,
                "ms_per_step": round(rm["elapsed"] / args.steps * 1e3, 4), "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f32 (FP64 state and reductions)",
                "data": "synthetic",
-               "config": {"workload": "256x256x160 compressible MHD, mixed precision, Ra=1e6, Q=1000, "
-                                      "T/Bz noise 1e-3, fixed dt 5e-5, diagnostics every 100 steps",
-                          "nx": NX, "ny": NY, "nz": NZ, "dt": DT, "ndu06": NDU06, "precision": "mixed"},
-               "roofline": roofline_of(rm, args.kernel, 4), "cpu_baseline": None,
+               "config": {"workload": desc + ", mixed precision, diagnostics every 100 steps",
+                          "nx": nx, "ny": ny, "nz": nz, "dt": dt, "ndu06": NDU06, "precision": "mixed"},
+               "roofline": roofline_of(rm, args.kernel, 4, args.workload), "cpu_baseline": None,
                "device_bytes": rm["device_bytes"]}
+        rm["solver"].close()
         print(json.dumps(out))
         return 0
-    r64 = run("fp64", args.steps, args.warmup, args.kernel)
+    r64 = run("fp64", args.steps, args.warmup, args.kernel, wl)
     value = cells * args.steps / r64["elapsed"] / 1e6
-    finite = all(float(np.abs(v).max()) < 1e30 for v in r64["state"].values())
     d64 = r64["diag"]
 
     mixed = None
     if args.precision == "both":
-        rm = run("mixed", args.steps, args.warmup, args.kernel)
+        rm = run("mixed", args.steps, args.warmup, args.kernel, wl)
         vm = cells * args.steps / rm["elapsed"] / 1e6
-        dm = rm["diag"]
-        # error of the mixed run against the FP64 run after the same steps
-        # (the FP64 path matches the FP64 oracle to 1e-10 in the parity tests)
-        fe = {n: float(np.abs(rm["state"][n] - r64["state"][n]).max() /
-                       max(np.abs(r64["state"][n]).max(), 1e-300)) for n in r64["state"]}
-        qe = {k: (float(abs(dm[k] - d64[k]) / abs(d64[k])) if d64.get(k) else None)
+        # error of the mixed run against the FP64 run after err_steps steps of
+        # both (the FP64 path matches the FP64 oracle to 1e-10 in the parity tests)
+        nerr = max(args.err_steps, r64["done"])
+        e64 = advance(r64, nerr, dt)
+        em = advance(rm, nerr, dt)
+        s64, sm = r64["solver"].get_state(), rm["solver"].get_state()
+        bscale = max(float(np.abs(s64[n]).max()) for n in ("bx", "by", "bz"))
+        fe = {n: float(np.abs(sm[n] - s64[n]).max() /
+                       (bscale if n in ("bx", "by", "bz") else max(float(np.abs(s64[n]).max()), 1e-300)))
+              for n in s64}
+        qe = {k: (float(abs(em[k] - e64[k]) / abs(e64[k])) if e64.get(k) else None)
               for k in ("nusselt", "kinenergy", "magenergy")}
         mixed = {"value": round(vm, 2), "ms_per_step": round(rm["elapsed"] / args.steps * 1e3, 4),
                  "speedup_vs_fp64": round(vm / value, 3),
-                 "rel_err_vs_fp64": {**{k: (float(f"{v:.3e}") if v is not None else None)
+                 "rel_err_vs_fp64": {"after_steps": nerr,
+                                     **{k: (float(f"{v:.3e}") if v is not None else None)
                                         for k, v in qe.items()},
                                      "fields": {k: float(f"{v:.3e}") for k, v in fe.items()}},
-                 "roofline": roofline_of(rm, args.kernel, 4),
+                 "roofline": roofline_of(rm, args.kernel, 4, args.workload),
                  "device_bytes": rm["device_bytes"]}
+        rm["solver"].close()
+    st = r64["solver"].get_state()
+    finite = all(float(np.abs(v).max()) < 1e30 for v in st.values())
+    r64["solver"].close()
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(workload_params())
+        cpu = cpu_baseline(workload_params()) if args.workload == "c2" else None
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(r64["elapsed"] / args.steps * 1e3, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "256x256x160 compressible MHD, FP64 plain, Ra=1e6, Q=1000, "
-                                  "T/Bz noise 1e-3, fixed dt 5e-5, diagnostics every 100 steps",
-                      "nx": NX, "ny": NY, "nz": NZ, "dt": DT, "ndu06": NDU06,
+           "config": {"workload": desc + ", FP64 plain method, diagnostics every 100 steps",
+                      "nx": nx, "ny": ny, "nz": nz, "dt": dt, "ndu06": NDU06,
                       "precision": "fp64"},
-           "roofline": roofline_of(r64, args.kernel, 8), "cpu_baseline": cpu,
+           "roofline": roofline_of(r64, args.kernel, 8, args.workload), "cpu_baseline": cpu,
            "mixed": mixed,
            "device_bytes": r64["device_bytes"],
            "diag": {"nusselt": d64.get("nusselt"), "kinenergy": d64.get("kinenergy"),
