@@ -368,6 +368,7 @@ MagArgs margs(mhd_ctx* c)
     if (c->su) { a.bxm = c->bxbar; a.bym = c->bybar; }
     if (c->eta_on) { a.jx = c->jx; a.jy = c->jy; a.jz = c->jz; }
     a.wf32 = c->wf32;
+    a.zc = c->hzc;
     return a;
 }
 

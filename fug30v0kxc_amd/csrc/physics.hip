@@ -216,27 +216,63 @@ __global__ void __launch_bounds__(1024) mean_emf_kernel(MagArgs a)
     }
 }
 
-// poldot, tordot, src/magnetic.f90:429-479
+// poldot, tordot, src/magnetic.f90:429-479.  Column walk: a thread owns one
+// (i,j) column and a.zc planes; pol and tor stay in 5-plane register windows
+// so each plane is read once; the two planes next to each wall use the
+// one-sided d2bydz rows (flag 3) with direct loads.
 template <typename TW>
-__global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs a)
+__global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
 {
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
-    if (q >= a.g.n3) return;
+    (void)av;
+    const MagArgs& a = kernarg<MagArgs>();
     const int nx = a.g.nx, nxy = a.g.nxy, nz = a.g.nz;
-    const int k = (int)(q / nxy);
-    const long col = q - (long)k * nxy;
+    const long col = (long)blockIdx.x * NT + threadIdx.x;
+    if (col >= nxy) return;
+    const int k0 = blockIdx.y * a.zc;
+    const int k1 = (k0 + a.zc < nz) ? k0 + a.zc : nz;
     const int jq = (int)(col / nx), ip = (int)(col - (long)jq * nx);
     const double k2 = a.akx2[ip] + a.aky2[jq];
     const bool dc = (ip == 0 && jq == 0);
-    const double il_c = dc ? 0.0 : (double)((const TW*)a.cc)[q] / k2;
-    const double il_d = dc ? 0.0 : (double)((const TW*)a.dz)[q] / k2;
     const double* pol = a.pol;
     const double* tor = a.tor;
-    const double pzz = dz2<double>(k, nz, 3, a.p.sc1, [&](int kk) { return pol[col + (long)kk * nxy]; });
-    const double tzz = dz2<double>(k, nz, 3, a.p.sc1, [&](int kk) { return tor[col + (long)kk * nxy]; });
-    const double zck = a.p.zck;
-    ((TW*)a.poldot)[q] = (TW)(zck * pol[q] * k2 + zck * pzz + il_c);
-    ((TW*)a.tordot)[q] = (TW)(zck * tor[q] * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q]);
+    const double zck = a.p.zck, s = a.p.sc1;
+    double pw[5], tw[5];
+    bool primed = false;
+    for (int k = k0; k < k1; ++k) {
+        const long q = col + (long)k * nxy;
+        double pzz, tzz, p0, t0;
+        if (k >= 2 && k <= nz - 3) {
+            if (!primed) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    pw[j + 1] = pol[q + (long)(j - 2) * nxy];
+                    tw[j + 1] = tor[q + (long)(j - 2) * nxy];
+                }
+                primed = true;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                pw[j] = pw[j + 1];
+                tw[j] = tw[j + 1];
+            }
+            pw[4] = pol[q + 2L * nxy];
+            tw[4] = tor[q + 2L * nxy];
+            pzz = s * (-pw[0] + 16.0 * pw[1] - 30.0 * pw[2] + 16.0 * pw[3] - pw[4]);
+            tzz = s * (-tw[0] + 16.0 * tw[1] - 30.0 * tw[2] + 16.0 * tw[3] - tw[4]);
+            p0 = pw[2];
+            t0 = tw[2];
+        } else {
+            pzz = dz2<double>(k, nz, 3, s, [&](int kk) { return pol[col + (long)kk * nxy]; });
+            tzz = dz2<double>(k, nz, 3, s, [&](int kk) { return tor[col + (long)kk * nxy]; });
+            p0 = pol[q];
+            t0 = tor[q];
+            primed = false;
+        }
+        const double il_c = dc ? 0.0 : (double)((const TW*)a.cc)[q] / k2;
+        const double il_d = dc ? 0.0 : (double)((const TW*)a.dz)[q] / k2;
+        ((TW*)a.poldot)[q] = (TW)(zck * p0 * k2 + zck * pzz + il_c);
+        ((TW*)a.tordot)[q] = (TW)(zck * t0 * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q]);
+    }
 }
 
 // spectral B and J pieces of bandj, src/magnetic.f90:519-617
@@ -279,31 +315,63 @@ __global__ void __launch_bounds__(NT) bandj_spec_kernel(MagArgs a)
 }
 
 // J^2, src/magnetic.f90:630-645 (dybzh/dxbzh/jzh hold physical-space fields here)
+// J^2 (src/magnetic.f90:596-645) and, for the eta(z) extension, J itself.
+// Column walk with 5-plane windows of bx, by for their d/dz (flag 5).
 template <typename TW>
-__global__ void __launch_bounds__(NT) jsq_kernel(MagArgs a)
+__global__ void __launch_bounds__(NT) jsq_kernel(MagArgs av)
 {
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
-    if (q >= a.g.n3) return;
+    (void)av;
+    const MagArgs& a = kernarg<MagArgs>();
     const int nxy = a.g.nxy, nz = a.g.nz;
-    const int k = (int)(q / nxy);
-    const long col = q - (long)k * nxy;
+    const long col = (long)blockIdx.x * NT + threadIdx.x;
+    if (col >= nxy) return;
+    const int k0 = blockIdx.y * a.zc;
+    const int k1 = (k0 + a.zc < nz) ? k0 + a.zc : nz;
     const TW* by = (const TW*)a.by;
     const TW* bx = (const TW*)a.bx;
-    double wk1 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)by[col + (long)kk * nxy]; });
-    double wk2 = dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return (double)bx[col + (long)kk * nxy]; });
-    if (a.bym) {
-        // mixed: bx, by hold no means; d/dz of the means (flag 5) is added back
-        wk1 += dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return a.bym[kk]; });
-        wk2 += dz1<double>(k, nz, 5, a.p.ho12, [&](int kk) { return a.bxm[kk]; });
-    }
-    const double ex = (double)((const TW*)a.dybzh)[q];
-    const double ey = (double)((const TW*)a.dxbzh)[q];
-    const double ez = (double)((const TW*)a.jzh)[q];
-    ((TW*)a.jsq)[q] = (TW)(ez * ez + ((ex - wk1) * (ex - wk1)) + ((wk2 - ey) * (wk2 - ey)));
-    if (a.jx) {
-        ((TW*)a.jx)[q] = (TW)(ex - wk1);
-        ((TW*)a.jy)[q] = (TW)(wk2 - ey);
-        ((TW*)a.jz)[q] = (TW)ez;
+    const double h = a.p.ho12;
+    double yw[5], xw[5];
+    bool primed = false;
+    for (int k = k0; k < k1; ++k) {
+        const long q = col + (long)k * nxy;
+        double wk1, wk2;
+        if (k >= 2 && k <= nz - 3) {
+            if (!primed) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    yw[j + 1] = (double)by[q + (long)(j - 2) * nxy];
+                    xw[j + 1] = (double)bx[q + (long)(j - 2) * nxy];
+                }
+                primed = true;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                yw[j] = yw[j + 1];
+                xw[j] = xw[j + 1];
+            }
+            yw[4] = (double)by[q + 2L * nxy];
+            xw[4] = (double)bx[q + 2L * nxy];
+            wk1 = h * (yw[0] - 8.0 * yw[1] + 8.0 * yw[3] - yw[4]);
+            wk2 = h * (xw[0] - 8.0 * xw[1] + 8.0 * xw[3] - xw[4]);
+        } else {
+            wk1 = dz1<double>(k, nz, 5, h, [&](int kk) { return (double)by[col + (long)kk * nxy]; });
+            wk2 = dz1<double>(k, nz, 5, h, [&](int kk) { return (double)bx[col + (long)kk * nxy]; });
+            primed = false;
+        }
+        if (a.bym) {
+            // mixed: bx, by hold no means; d/dz of the means (flag 5) is added back
+            wk1 += dz1<double>(k, nz, 5, h, [&](int kk) { return a.bym[kk]; });
+            wk2 += dz1<double>(k, nz, 5, h, [&](int kk) { return a.bxm[kk]; });
+        }
+        const double ex = (double)((const TW*)a.dybzh)[q];
+        const double ey = (double)((const TW*)a.dxbzh)[q];
+        const double ez = (double)((const TW*)a.jzh)[q];
+        ((TW*)a.jsq)[q] = (TW)(ez * ez + ((ex - wk1) * (ex - wk1)) + ((wk2 - ey) * (wk2 - ey)));
+        if (a.jx) {
+            ((TW*)a.jx)[q] = (TW)(ex - wk1);
+            ((TW*)a.jy)[q] = (TW)(wk2 - ey);
+            ((TW*)a.jz)[q] = (TW)ez;
+        }
     }
 }
 
@@ -388,7 +456,8 @@ int launch_mean_emf(const MagArgs& a, hipStream_t s)
 }
 int launch_mag_rhs(const MagArgs& a, hipStream_t s)
 {
-    MHD_LAUNCH_W(mag_rhs_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    if (a.zc <= 0) return -10;
+    MHD_LAUNCH_W(mag_rhs_kernel, dim3(nblk(a.g.nxy), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_bandj_spec(const MagArgs& a, hipStream_t s)
@@ -398,7 +467,8 @@ int launch_bandj_spec(const MagArgs& a, hipStream_t s)
 }
 int launch_jsq(const MagArgs& a, hipStream_t s)
 {
-    MHD_LAUNCH_W(jsq_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    if (a.zc <= 0) return -10;
+    MHD_LAUNCH_W(jsq_kernel, dim3(nblk(a.g.nxy), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out)

@@ -88,6 +88,7 @@ struct MagArgs {
     void *jsq;
     void *jx, *jy, *jz;           // eta(z): J kept for the EMF, or null
     int wf32;
+    int zc;                       // planes per thread of the column-walking kernels
 };
 int launch_mag_spec1(const MagArgs& a, hipStream_t s);   // D, C from E-hat
 int launch_mean_emf(const MagArgs& a, hipStream_t s);    // mean-field RHS
