@@ -23,7 +23,7 @@ from ._abi import (FIELD_IDS, HISTORY_FIELDS, KERNELS, MHD_PREC_FP64, MHD_PREC_M
                    OPT_CONSTFLUX, OPT_DYNAMO, OPT_KINEMATIC, OPT_PERFECT, OPT_RADBC,
                    VECTOR_FIELDS, MhdDiag, MhdParams, MhdScalars)
 
-__all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "write_dump", "read_dump", "expand", "box_filter", "MHD_PREC_FP64",
+__all__ = ["Solver", "MhdError", "ra_q_to_ck_f", "reference_params", "write_dump", "read_dump", "expand", "box_filter", "Ensemble", "MHD_PREC_FP64",
            "MHD_PREC_MIXED"]
 
 
@@ -334,3 +334,51 @@ class Solver:
 
     def device_bytes(self):
         return self._lib.mhd_device_bytes(self._ctx)
+
+
+class Ensemble:
+    """Independent boxes (Solver objects on this GPU) stepped together: the
+    kernels of `block` steps of every box are captured once into one
+    hipGraph (each box on its own stream branch) and replayed.  Results
+    equal Solver.step(n, dt) per box.  A box must not be stepped on its own
+    while it belongs to an ensemble."""
+
+    def __init__(self, solvers, dt):
+        self._lib = _abi.load()
+        self._e = None
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[sv._ctx.value for sv in self.solvers])
+        e = ctypes.c_void_p()
+        code = self._lib.mhd_ensemble_create(arr, len(self.solvers), float(dt), ctypes.byref(e))
+        if code:
+            raise MhdError(code, "mhd_ensemble_create")
+        self._e = e
+        self.dt = dt
+        self.block = self._lib.mhd_ensemble_block()
+
+    @property
+    def nodes(self):
+        return self._lib.mhd_ensemble_nodes(self._e)
+
+    def step(self, nsteps):
+        code = self._lib.mhd_ensemble_step(self._e, int(nsteps))
+        if code:
+            raise MhdError(code, "mhd_ensemble_step")
+
+    def close(self):
+        if self._e is not None:
+            self._lib.mhd_ensemble_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+

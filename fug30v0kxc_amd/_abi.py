@@ -119,6 +119,11 @@ def load():
         "mhd_device_bytes": (ctypes.c_long, [vp]),
         "mhd_write_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp), i]),
         "mhd_read_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp)]),
+        "mhd_ensemble_create": (i, [ctypes.POINTER(vp), i, d, ctypes.POINTER(vp)]),
+        "mhd_ensemble_step": (i, [vp, i]),
+        "mhd_ensemble_block": (i, []),
+        "mhd_ensemble_nodes": (ctypes.c_long, [vp]),
+        "mhd_ensemble_destroy": (None, [vp]),
         "mhd_expand": (i, [i, i, i, dp, dp]),
         "mhd_box_filter": (i, [i, i, i, i, dp, dp]),
     }

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <array>
 #include <vector>
 
 #include "physics.hpp"
@@ -1355,6 +1356,143 @@ int mhd_step(mhd_ctx_t* c, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* l
         if (dt_fixed <= 0.0 && S.nit % 10 == 0) TRY(run_calcdt(c, &S.dt));
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent boxes stepped by one hipGraph launch per block of
+// ENS_BLOCK steps.  With a fixed dt the AB3 coefficients are constant after
+// the Euler and AB2 start-up steps, the history ring has period 3 and the
+// dealias cadence period 10, so the kernel sequence of 30 steps repeats
+// exactly.  It is captured once, every box on its own stream branch
+// (fork/join through events), and replayed.
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct mhd_ensemble {
+    std::vector<mhd_ctx*> box;
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double dt = 0.0;
+    long nodes = 0;
+};
+
+namespace {
+constexpr int ENS_BLOCK = 30;
+
+bool ab3_steady(const mhd_ctx* c, double dt)
+{
+    return !c->sc.first && !c->sc.second && c->sc.dt == dt && c->sc.dt1 == dt && c->sc.dt2 == dt + dt;
+}
+
+void ens_free(mhd_ensemble* e)
+{
+    if (e->s) (void)hipStreamSynchronize(e->s);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (hipEvent_t ev : e->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->s) (void)hipStreamDestroy(e->s);
+    delete e;
+}
+}  // namespace
+
+extern "C" {
+
+int mhd_ensemble_create(mhd_ctx_t* const* boxes, int nbox, double dt, mhd_ensemble_t** out)
+{
+    if (!boxes || nbox < 1 || !(dt > 0.0) || !out) return MHD_E_ARG;
+    *out = nullptr;
+    for (int b = 0; b < nbox; ++b) {
+        if (!boxes[b]) return MHD_E_ARG;
+        if (!boxes[b]->have_state) return MHD_E_STATE;
+        for (int b2 = 0; b2 < b; ++b2)
+            if (boxes[b2] == boxes[b]) return MHD_E_ARG;
+    }
+    // start-up steps until the AB3 coefficients are those of a constant dt
+    for (int b = 0; b < nbox; ++b) {
+        mhd_ctx* c = boxes[b];
+        for (int n = 0; n < 4 && !ab3_steady(c, dt); ++n) TRY(mhd_step(c, 1, dt, 0, nullptr));
+        if (!ab3_steady(c, dt)) return MHD_E_STATE;
+        c->timed_kernel = MHD_K_NONE;
+        TRY(mhd_sync(c));
+    }
+    mhd_ensemble* e = new mhd_ensemble();
+    e->box.assign(boxes, boxes + nbox);
+    e->dt = dt;
+    e->join.assign(nbox, nullptr);
+    auto fail = [&](int code) {
+        ens_free(e);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
+        return fail(MHD_E_DEVICE);
+    for (int b = 0; b < nbox; ++b)
+        if (hipEventCreateWithFlags(&e->join[b], hipEventDisableTiming) != hipSuccess) return fail(MHD_E_DEVICE);
+
+    // capture: the host-side bookkeeping of the steps runs on copies that are
+    // put back afterwards (nothing executes during capture)
+    std::vector<mhd_scalars_t> keep(nbox);
+    std::vector<std::array<int, 3>> ring(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        keep[b] = e->box[b]->sc;
+        ring[b] = {e->box[b]->ring[0], e->box[b]->ring[1], e->box[b]->ring[2]};
+    }
+    if (hipStreamBeginCapture(e->s, hipStreamCaptureModeRelaxed) != hipSuccess) return fail(MHD_E_DEVICE);
+    int rc = 0;
+    if (hipEventRecord(e->fork, e->s) != hipSuccess) rc = MHD_E_DEVICE;
+    for (int b = 0; b < nbox && !rc; ++b) {
+        mhd_ctx* c = e->box[b];
+        if (hipStreamWaitEvent(c->s, e->fork, 0) != hipSuccess) { rc = MHD_E_DEVICE; break; }
+        rc = mhd_step(c, ENS_BLOCK, dt, 0, nullptr);
+        if (!rc && hipEventRecord(e->join[b], c->s) != hipSuccess) rc = MHD_E_DEVICE;
+        if (!rc && hipStreamWaitEvent(e->s, e->join[b], 0) != hipSuccess) rc = MHD_E_DEVICE;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->s, &g);
+    e->graph = g;
+    for (int b = 0; b < nbox; ++b) {
+        e->box[b]->sc = keep[b];
+        for (int l = 0; l < 3; ++l) e->box[b]->ring[l] = ring[b][l];
+    }
+    if (rc) return fail(rc);
+    if (ce != hipSuccess || !g) return fail(MHD_E_DEVICE);
+    size_t nn = 0;
+    if (hipGraphGetNodes(g, nullptr, &nn) == hipSuccess) e->nodes = (long)nn;
+    if (hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess) return fail(MHD_E_DEVICE);
+    *out = e;
+    return MHD_OK;
+}
+
+int mhd_ensemble_step(mhd_ensemble_t* e, int nsteps)
+{
+    if (!e || nsteps < 0 || nsteps % ENS_BLOCK) return MHD_E_ARG;
+    const int blocks = nsteps / ENS_BLOCK;
+    for (int i = 0; i < blocks; ++i) MHD_HIP_CHECK(hipGraphLaunch(e->exec, e->s));
+    MHD_HIP_CHECK(hipStreamSynchronize(e->s));
+    MHD_HIP_CHECK(hipGetLastError());
+    // the host-side driver scalars advance as mhd_step would advance them
+    for (mhd_ctx* c : e->box) {
+        for (int n = 0; n < nsteps; ++n) {
+            c->sc.nit += 1;
+            c->sc.iter += 1;
+            c->sc.ttime += c->sc.dt;
+        }
+    }
+    return MHD_OK;
+}
+
+int mhd_ensemble_block(void) { return ENS_BLOCK; }
+
+long mhd_ensemble_nodes(mhd_ensemble_t* e) { return e ? e->nodes : 0; }
+
+void mhd_ensemble_destroy(mhd_ensemble_t* e)
+{
+    if (e) ens_free(e);
 }
 
 int mhd_global_averages(mhd_ctx_t* c, double out[4])

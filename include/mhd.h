@@ -189,6 +189,23 @@ typedef struct mhd_dump {
 int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const fields[8], int iprec);
 int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8]);
 
+/* ---- ensembles of independent boxes (no counterpart in the reference) ------
+ * The boxes (contexts with a state, any sizes, parameters and precisions on
+ * this GPU) are brought to steady AB3 coefficients at the fixed step dt (up
+ * to two start-up steps each), then the kernels of MHD_ENSEMBLE_BLOCK steps
+ * of every box are captured into one hipGraph, each box on its own stream
+ * branch.  mhd_ensemble_step launches it nsteps / block times (nsteps a
+ * multiple of the block) and advances each box's driver scalars; results
+ * equal mhd_step(box, nsteps, dt, 0, NULL) per box.  Diagnostics are taken
+ * per box between calls.  A box must not be stepped on its own while in an
+ * ensemble. */
+typedef struct mhd_ensemble mhd_ensemble_t;
+int mhd_ensemble_create(mhd_ctx_t* const* boxes, int nbox, double dt, mhd_ensemble_t** out);
+int mhd_ensemble_step(mhd_ensemble_t* e, int nsteps);
+int mhd_ensemble_block(void);
+long mhd_ensemble_nodes(mhd_ensemble_t* e);   /* graph nodes (kernels and joins) */
+void mhd_ensemble_destroy(mhd_ensemble_t* e);
+
 /* ---- resampling and coarse-graining (host arrays in, host arrays out) ---- */
 /* expand (interpolate/fft.F90:832-927, called per field at
  * interpolate/main.F90:299-306): a (nx,ny,nz) field becomes (2nx,2ny,nz) by
