@@ -35,6 +35,7 @@ DT = 5e-5
 NDU06 = 100
 # the mixed-precision error is measured after this many steps of both modes
 ERR_STEPS = 2000
+AUTOTUNE = True
 
 # (nx, ny, nz, dt, extra parameters, description)
 WORKLOADS = {
@@ -90,6 +91,7 @@ def run(precision, steps, warmup, kernel, wl):
     prec = MHD_PREC_FP64 if precision == "fp64" else MHD_PREC_MIXED
     s = Solver(nx, ny, nz, workload_params(extra), precision=prec)
     s.static()
+    tune = s.autotune() if AUTOTUNE else None
     s.step(warmup, dt, NDU06)
     s.sync()
     s.kernel_timing(kernel, True)
@@ -100,8 +102,71 @@ def run(precision, steps, warmup, kernel, wl):
     t1 = time.perf_counter()
     kms, klaunch = s.kernel_time()
     s.kernel_timing(kernel, False)
-    return {"solver": s, "elapsed": t1 - t0, "diag": diag, "kms": kms, "klaunch": klaunch,
+    return {"solver": s, "elapsed": t1 - t0, "diag": diag, "kms": kms, "klaunch": klaunch, "tune": tune,
             "cells": nx * ny * nz, "device_bytes": s.device_bytes(), "done": warmup + steps}
+
+
+def ensemble_bench(steps, seed=20261202):
+    """32 seeded independent boxes (nx = ny in {64, 96, 128}, nz in {48, 64, 96},
+    Ra and Q log-uniform in [1e4, 1e6] and [10, 1e3], eta(z) contrast
+    log-uniform in [1, 100] at z = 0.5), mixed precision, each autotuned;
+    `steps` steps of all boxes by hipGraph replays against the same boxes
+    stepped one after another."""
+    import numpy as np
+    from fug30v0kxc_amd import MHD_PREC_MIXED, Ensemble, Solver, reference_params
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(32):
+        n = int(rng.choice([64, 96, 128]))
+        nz = int(rng.choice([48, 64, 96]))
+        ra = 10 ** rng.uniform(4, 6)
+        q = 10 ** rng.uniform(1, 3)
+        ec = 10 ** rng.uniform(0, 2)
+        p = reference_params(ra=ra, q=q, t_amp=1e-3, b_amp=1e-3)
+        p.update(eta_c=ec, eta_z0=0.5, eta_w=0.05)
+        specs.append((n, nz, p))
+    dt = 1e-6
+    cells = sum(n * n * nz for n, nz, _ in specs)
+
+    def make(tunes=None):
+        out = []
+        for i, (n, nz, p) in enumerate(specs):
+            s = Solver(n, n, nz, p, precision=MHD_PREC_MIXED)
+            s.static()
+            if tunes is None:
+                s.tune = s.autotune()
+            else:
+                s.set_tune(**{k: v for k, v in tunes[i].items() if not k.startswith("ms_")})
+            out.append(s)
+        return out
+    ens_boxes = make()
+    t0 = time.perf_counter()
+    ens = Ensemble(ens_boxes, dt)
+    t_capture = time.perf_counter() - t0
+    ens.step(ens.block)                      # first replay (untimed)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    t_graph = time.perf_counter() - t0
+    nodes = ens.nodes
+    ens.close()
+    seq = make([b.tune for b in ens_boxes])     # the same launch shapes, so bit-identical
+    for s in seq:
+        s.step(2 + ens.block, dt, 0)
+        s.sync()
+    t0 = time.perf_counter()
+    for s in seq:
+        s.step(steps, dt, 0)
+    for s in seq:
+        s.sync()
+    t_seq = time.perf_counter() - t0
+    same = all(np.array_equal(a.get_state()["t"], b.get_state()["t"]) for a, b in zip(ens_boxes, seq))
+    for s in ens_boxes + seq:
+        s.close()
+    return {"boxes": len(specs), "cells": cells, "steps": steps, "dt": dt,
+            "graph_mcell_steps_per_s": round(cells * steps / t_graph / 1e6, 2),
+            "sequential_mcell_steps_per_s": round(cells * steps / t_seq / 1e6, 2),
+            "graph_speedup": round(t_seq / t_graph, 3), "graph_nodes": nodes,
+            "capture_s": round(t_capture, 2), "identical_to_sequential": bool(same)}
 
 
 def advance(r, total, dt):
@@ -153,8 +218,10 @@ def main():
                          "(e.g. for a kernel profile of one mode)")
     ap.add_argument("--kernel", default="xderiv2", choices=("xderiv2", "yderiv2"),
                     help="kernel family for the roofline object")
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS),
-                    help="c2: 256x256x160 (default); c3: 512x512x256 with the eta(z) step")
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4"],
+                    help="c2: 256x256x160 (default); c3: 512x512x256 with the eta(z) step; "
+                         "c4: ensemble of 32 boxes in hipGraph replays")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--err-steps", type=int, default=ERR_STEPS,
                     help="steps after which the mixed run is compared with the FP64 run")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -176,6 +243,20 @@ def main():
         return 2
 
     import numpy as np
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
+    if args.workload == "c4":
+        steps = max(30, (args.steps // 30) * 30)
+        r = ensemble_bench(steps)
+        out = {"metric": METRIC, "value": r["graph_mcell_steps_per_s"], "unit": "Mcell-steps/s",
+               "n_gpus": 1, "steps": steps, "warmup": 32, "ms_per_step": None,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32 (FP64 state and reductions)", "data": "synthetic",
+               "config": {"workload": "ensemble of 32 seeded boxes, mixed precision, hipGraph replays",
+                          **{k: v for k, v in r.items() if k not in ("graph_mcell_steps_per_s",)}},
+               "roofline": None, "cpu_baseline": None}
+        print(json.dumps(out))
+        return 0
     wl = WORKLOADS[args.workload]
     nx, ny, nz, dt, extra, desc = wl
     cells = nx * ny * nz
@@ -240,6 +321,7 @@ def main():
                       "precision": "fp64"},
            "roofline": roofline_of(r64, args.kernel, 8, args.workload), "cpu_baseline": cpu,
            "mixed": mixed,
+           "tune": r64.get("tune"),
            "device_bytes": r64["device_bytes"],
            "diag": {"nusselt": d64.get("nusselt"), "kinenergy": d64.get("kinenergy"),
                     "magenergy": d64.get("magenergy"), "finite": finite}}

@@ -73,6 +73,11 @@ class MhdDump(ctypes.Structure):
                 + [("history", ctypes.c_char * 256)])
 
 
+class MhdTune(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int) for n in ("variant_x", "variant_y", "mult_x", "mult_y", "zc")]
+                + [(n, ctypes.c_double) for n in ("ms_x", "ms_y", "ms_col")])
+
+
 _lib = None
 
 
@@ -119,6 +124,8 @@ def load():
         "mhd_device_bytes": (ctypes.c_long, [vp]),
         "mhd_write_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp), i]),
         "mhd_read_dump": (i, [ctypes.c_char_p, ctypes.POINTER(MhdDump), ctypes.POINTER(dp)]),
+        "mhd_autotune": (i, [vp, ctypes.POINTER(MhdTune)]),
+        "mhd_set_tune": (i, [vp, ctypes.POINTER(MhdTune)]),
         "mhd_ensemble_create": (i, [ctypes.POINTER(vp), i, d, ctypes.POINTER(vp)]),
         "mhd_ensemble_step": (i, [vp, i]),
         "mhd_ensemble_block": (i, []),

@@ -68,6 +68,18 @@ struct Plan {
     static constexpr int P = sizeof...(Rs);
 };
 
+// Alternative plan per length for the autotuner (type void: none).  Each
+// trades LDS passes against butterfly size (registers, occupancy).
+template <int N>
+struct AltPlan {
+    using type = void;
+};
+template <> struct AltPlan<64> { using type = Plan<64, 8, 8>; };
+template <> struct AltPlan<128> { using type = Plan<128, 8, 16>; };
+template <> struct AltPlan<160> { using type = Plan<160, 4, 8, 5>; };
+template <> struct AltPlan<256> { using type = Plan<256, 4, 8, 8>; };
+template <> struct AltPlan<512> { using type = Plan<512, 8, 8, 8>; };
+
 template <typename PL>
 constexpr int plan_radix(int p, bool inv)
 {

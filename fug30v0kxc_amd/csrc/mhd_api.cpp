@@ -77,6 +77,8 @@ struct mhd_ctx {
     double *red_partial, *red_result, *d_bzbar;
     double* stage;     // device FP64 staging (one field)
     int ncu;           // compute units (persistent transform grids)
+    int spec_var[2];   // radix plan per direction (0 default, 1 alternative), tuned
+    int spec_mult[2];  // persistent-grid multiple per direction, tuned
     int hzc;           // planes per thread of the column-walking hydro kernel
     double* hbuf;      // pinned host staging (one field)
 
@@ -225,6 +227,8 @@ SpecLaunch spec_base(mhd_ctx* c, int dir, int mode, bool f64 = false)
     else L.tw = dir == 0 ? c->twx : c->twy;
     L.icut = dir == 0 ? c->icutx : c->icuty;
     L.ncu = c->ncu;
+    L.variant = c->spec_var[dir];
+    L.grid_mult = c->spec_mult[dir];
     return L;
 }
 
@@ -876,6 +880,8 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->ring[0] = 0; c->ring[1] = 1; c->ring[2] = 2;
     c->sc.first = 1;
     c->ncu = ncu;
+    c->spec_var[0] = c->spec_var[1] = 0;
+    c->spec_mult[0] = c->spec_mult[1] = 1;
     {
         // column-walk chunks: about 2048 workgroups of 256 columns, >= 8 planes each
         const long bx = (c->nxy + 255) / 256;
@@ -1487,6 +1493,115 @@ int mhd_ensemble_step(mhd_ensemble_t* e, int nsteps)
 }
 
 int mhd_ensemble_block(void) { return ENS_BLOCK; }
+
+// ---------------------------------------------------------------------------
+// Autotuner: the launch shapes of this box timed on this GPU
+// ---------------------------------------------------------------------------
+}  // extern "C"
+namespace {
+// average milliseconds of `reps` calls of f on the context's stream
+template <typename F>
+int time_ms(mhd_ctx* c, int reps, F&& f, double* ms)
+{
+    hipEvent_t a, b;
+    MHD_HIP_CHECK(hipEventCreate(&a));
+    MHD_HIP_CHECK(hipEventCreate(&b));
+    int rc = f();   // warm-up
+    if (!rc) {
+        (void)hipEventRecord(a, c->s);
+        for (int r = 0; r < reps && !rc; ++r) rc = f();
+        (void)hipEventRecord(b, c->s);
+        (void)hipEventSynchronize(b);
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, a, b);
+        *ms = (double)t / reps;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+}  // namespace
+extern "C" {
+
+int mhd_set_tune(mhd_ctx_t* c, const mhd_tune_t* t)
+{
+    TRY(check_ctx(c));
+    if (!t) return MHD_E_ARG;
+    if (t->variant_x < 0 || t->variant_x > 1 || t->variant_y < 0 || t->variant_y > 1) return MHD_E_ARG;
+    if ((t->variant_x && !spectral_has_variant(c->nx)) || (t->variant_y && !spectral_has_variant(c->ny)))
+        return MHD_E_ARG;
+    if (t->mult_x < 1 || t->mult_x > 8 || t->mult_y < 1 || t->mult_y > 8 || t->zc < 1) return MHD_E_ARG;
+    c->spec_var[0] = t->variant_x;
+    c->spec_var[1] = t->variant_y;
+    c->spec_mult[0] = t->mult_x;
+    c->spec_mult[1] = t->mult_y;
+    c->hzc = t->zc;
+    return 0;
+}
+
+int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
+{
+    TRY(check_ctx(c));
+    if (!c->have_state) return MHD_E_STATE;
+    const int saved_timing = c->timed_kernel;
+    c->timed_kernel = MHD_K_NONE;
+    const int wf = c->wf32;
+    void** W = c->wk;
+    const void* in[4] = {c->su ? (const void*)c->st : c->t, c->su ? (const void*)c->su : c->u,
+                         c->su ? (const void*)c->sv : c->v, c->su ? (const void*)c->sw : c->w};
+    const int sf = c->su ? 1 : 0;
+    double best_ms[2] = {0, 0};
+    // transforms: radix plan x persistent-grid multiple, on the hydro d/dx, d2/dx2 set
+    for (int dir = 0; dir < 2; ++dir) {
+        const int n = dir == 0 ? c->nx : c->ny;
+        double best = 1e30;
+        int bv = 0, bm = 1;
+        for (int var = 0; var < (spectral_has_variant(n) ? 2 : 1); ++var)
+            for (int mult = 1; mult <= 2; ++mult) {
+                c->spec_var[dir] = var;
+                c->spec_mult[dir] = mult;
+                SpecLaunch L = spec_base(c, dir, SM_DERIV2);
+                const int o1[4] = {dir == 0 ? W_TX : W_TY, dir == 0 ? W_UX : W_UY, dir == 0 ? W_VX : W_VY,
+                                   dir == 0 ? W_WX : W_WY};
+                const int o2[4] = {dir == 0 ? W_TXX : W_TYY, dir == 0 ? W_UXX : W_UYY,
+                                   dir == 0 ? W_VXX : W_VYY, dir == 0 ? W_WXX : W_WYY};
+                for (int j = 0; j < 4; ++j) L.job[j] = sjob(in[j], sf, W[o1[j]], W[o2[j]], MUL_D1, MUL_D2, wf);
+                L.njobs = 4;
+                double ms = 0.0;
+                TRY(time_ms(c, 3, [&] { return spec(c, L); }, &ms));
+                if (ms < best) { best = ms; bv = var; bm = mult; }
+            }
+        c->spec_var[dir] = bv;
+        c->spec_mult[dir] = bm;
+        best_ms[dir] = best;
+    }
+    // column-walking kernels: planes per thread
+    double bestc = 1e30;
+    int bz = c->hzc;
+    const int nin = c->nz > 8 ? c->nz - 8 : 1;
+    const int cands[] = {8, 12, 16, 20, 24, 32, 48};
+    for (int zc : cands) {
+        if (zc > nin && zc != cands[0]) continue;
+        c->hzc = zc;
+        HydroColArgs a = colargs(c);
+        double ms = 0.0;
+        TRY(time_ms(c, 3, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); }, &ms));
+        if (ms < bestc) { bestc = ms; bz = zc; }
+    }
+    c->hzc = bz;
+    c->timed_kernel = saved_timing;
+    if (out) {
+        out->variant_x = c->spec_var[0];
+        out->variant_y = c->spec_var[1];
+        out->mult_x = c->spec_mult[0];
+        out->mult_y = c->spec_mult[1];
+        out->zc = c->hzc;
+        out->ms_x = best_ms[0];
+        out->ms_y = best_ms[1];
+        out->ms_col = bestc;
+    }
+    return mhd_sync(c);
+}
 
 long mhd_ensemble_nodes(mhd_ensemble_t* e) { return e ? e->nodes : 0; }
 

@@ -1,4 +1,7 @@
 // spectral.cpp -- dispatch of the horizontal transform kernels by length.
+#include <type_traits>
+
+#include "fft_reg.hpp"
 #include "spectral.hpp"
 
 namespace mhd {
@@ -16,6 +19,16 @@ bool spectral_len_supported(int n)
 {
     switch (n) {
 #define X(n_) case n_: return true;
+        MHD_ALL_SIZES(X)
+#undef X
+    }
+    return false;
+}
+
+bool spectral_has_variant(int n)
+{
+    switch (n) {
+#define X(n_) case n_: return !std::is_void<AltPlan<n_>::type>::value;
         MHD_ALL_SIZES(X)
 #undef X
     }

@@ -55,6 +55,8 @@ struct SpecLaunch {
     const void* tw;   // W_N^t table in the compute type
     int icut;         // dealias cut-off (n/2)/3
     int ncu;          // compute units: the persistent grid is occupancy x ncu workgroups
+    int grid_mult;    // ... times this (>= 1; 0 = 1), a tunable launch shape
+    int variant;      // radix plan: 0 the default, 1 the length's alternative (if any)
     int max_groups;   // optional further cap on the grid; 0 = none
 };
 
@@ -62,5 +64,7 @@ struct SpecLaunch {
 int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s);
 
 bool spectral_len_supported(int n);
+// the length has an alternative radix plan (SpecLaunch::variant = 1)
+bool spectral_has_variant(int n);
 
 }  // namespace mhd

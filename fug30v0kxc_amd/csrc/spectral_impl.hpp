@@ -417,7 +417,8 @@ static int launch_one(const SpecLaunch& L, hipStream_t s)
         per_cu = n;
     }
     long grid = items;
-    if (L.ncu > 0 && grid > (long)per_cu * L.ncu) grid = (long)per_cu * L.ncu;
+    const long resident = (long)per_cu * L.ncu * (L.grid_mult > 1 ? L.grid_mult : 1);
+    if (L.ncu > 0 && grid > resident) grid = resident;
     if (L.max_groups > 0 && grid > L.max_groups) grid = L.max_groups;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, s, L);
     return hipGetLastError() == hipSuccess ? 0 : -22;
@@ -442,14 +443,22 @@ static int launch_mode(const SpecLaunch& L, hipStream_t s)
     return -23;
 }
 
-template <typename TC, int N>
-static int launch_n(const SpecLaunch& L, hipStream_t s)
+template <typename TC, typename PL>
+static int launch_plan(const SpecLaunch& L, hipStream_t s)
 {
-    using PL = FftPlan<N>;
     if (L.dir == 0) return launch_mode<TC, PL, 0, spec_b<TC, PL, 0>()>(L, s);
     constexpr int B0 = spec_b<TC, PL, 1>();
     if (L.g.nx % (2 * B0) == 0) return launch_mode<TC, PL, 1, B0>(L, s);
     return launch_mode<TC, PL, 1, 1>(L, s);
+}
+
+template <typename TC, int N>
+static int launch_n(const SpecLaunch& L, hipStream_t s)
+{
+    if constexpr (!std::is_void<typename AltPlan<N>::type>::value) {
+        if (L.variant == 1) return launch_plan<TC, typename AltPlan<N>::type>(L, s);
+    }
+    return launch_plan<TC, FftPlan<N>>(L, s);
 }
 
 }  // namespace mhd

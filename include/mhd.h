@@ -189,6 +189,21 @@ typedef struct mhd_dump {
 int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const fields[8], int iprec);
 int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8]);
 
+/* ---- autotuner (no counterpart in the reference) ---------------------------
+ * Times this box's launch shapes on this GPU and keeps the fastest in the
+ * context: per transform direction the radix plan (default or the length's
+ * alternative) and the persistent-grid multiple (1 or 2 x resident), on the
+ * hydro d/dx, d2/dx2 launch; the planes per thread of the column-walking
+ * kernels.  Needs a state; overwrites only work arrays and the newest RHS
+ * level (recomputed by every step), so it may run between steps. */
+typedef struct mhd_tune {
+    int variant_x, variant_y, mult_x, mult_y, zc;
+    double ms_x, ms_y, ms_col;   /* best times of the timed launches */
+} mhd_tune_t;
+int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out);
+/* apply given choices (e.g. tuned on an earlier run); MHD_E_ARG if invalid */
+int mhd_set_tune(mhd_ctx_t* c, const mhd_tune_t* t);
+
 /* ---- ensembles of independent boxes (no counterpart in the reference) ------
  * The boxes (contexts with a state, any sizes, parameters and precisions on
  * this GPU) are brought to steady AB3 coefficients at the fixed step dt (up
