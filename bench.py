@@ -302,6 +302,7 @@ def main():
                                         for k, v in qe.items()},
                                      "fields": {k: float(f"{v:.3e}") for k, v in fe.items()}},
                  "roofline": roofline_of(rm, args.kernel, 4, args.workload),
+                 "tune": rm.get("tune"),
                  "device_bytes": rm["device_bytes"]}
         rm["solver"].close()
     st = r64["solver"].get_state()

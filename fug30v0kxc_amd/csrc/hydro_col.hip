@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(HNT) hydro_col_kernel(HydroColArgs av)
     };
 
     // ---- point-wise assembly at plane k, in the reference's order of accumulation
-    auto finish = [&](int k, const ZD& d) {
+    auto finish = [&](int k, const ZD& d) __attribute__((always_inline)) {
         const Phys& p = a.p;
         const long q = idx(k);
         const double rr = d.rr, tt = d.tt, u0 = d.u0, v0 = d.v0, w0 = d.w0;
@@ -180,11 +180,60 @@ __global__ void __launch_bounds__(HNT) hydro_col_kernel(HydroColArgs av)
         wdot = wdot - tt * rzc;
         // 487-497
         const double ri = 1.0 / rr;
-        ((TW*)a.rdot)[q] = (TW)rdot;
-        ((TW*)a.tdot)[q] = (TW)(tdot * ri);
-        ((TW*)a.udot)[q] = (TW)(udot * ri);
-        ((TW*)a.vdot)[q] = (TW)(vdot * ri);
-        ((TW*)a.wdot)[q] = (TW)(wdot * ri);
+        const TW fu = (TW)(udot * ri), fv = (TW)(vdot * ri), fw = (TW)(wdot * ri), fr = (TW)rdot,
+                 ft = (TW)(tdot * ri);
+        if (!a.ab3) {
+            ((TW*)a.rdot)[q] = fr;
+            ((TW*)a.tdot)[q] = ft;
+            ((TW*)a.udot)[q] = fu;
+            ((TW*)a.vdot)[q] = fv;
+            ((TW*)a.wdot)[q] = fw;
+            return;
+        }
+        // AB3, src/main.F90:413-443, in the order x + c0 f3 + c1 f2 + c2 f1
+        auto upd = [&](int i, double xo, TW f) {
+            return xo + a.c0 * (double)f + a.c1 * (double)((const TW*)a.h2[i])[q] +
+                   a.c2 * (double)((const TW*)a.h1[i])[q];
+        };
+        double xu, xv, xw, xr, xt;
+        if constexpr (MIX) {
+            xu = upd(0, a.x64[0][q], fu);
+            xv = upd(1, a.x64[1][q], fv);
+            xw = upd(2, a.x64[2][q], fw);
+            xr = upd(3, a.x64[3][q], fr);
+            xt = upd(4, a.x64[4][q], ft);
+        } else {
+            xu = upd(0, u0, fu);
+            xv = upd(1, v0, fv);
+            xw = upd(2, w0, fw);
+            xr = upd(3, rr, fr);
+            xt = upd(4, tt, ft);
+        }
+        TW rd = fr;
+        if (xr < 0.05) {
+            // density floor: clamp and zero its history
+            xr = 0.05;
+            rd = (TW)0;
+            ((TW*)a.h3z[0])[q] = (TW)0;
+            ((TW*)a.h3z[1])[q] = (TW)0;
+        }
+        ((TW*)a.udot)[q] = fu;
+        ((TW*)a.vdot)[q] = fv;
+        ((TW*)a.wdot)[q] = fw;
+        ((TW*)a.rdot)[q] = rd;
+        ((TW*)a.tdot)[q] = ft;
+        a.xnew[0][q] = xu;
+        a.xnew[1][q] = xv;
+        a.xnew[2][q] = xw;
+        a.xnew[3][q] = xr;
+        a.xnew[4][q] = xt;
+        if constexpr (MIX) {
+            a.snew[0][q] = (float)xu;
+            a.snew[1][q] = (float)xv;
+            a.snew[2][q] = (float)xw;
+            a.snew[3][q] = (float)(xr - a.r0[k]);
+            a.snew[4][q] = (float)(xt - a.t0[k]);
+        }
     };
 
     // boundary rows: the reference's row stencils with direct loads

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <array>
+#include <utility>
 #include <vector>
 
 #include "physics.hpp"
@@ -68,6 +69,10 @@ struct mhd_ctx {
     int h32;
     double* mhist[2][3];
     float *su, *sv, *sw, *st, *sr;   // mixed: FP32 fluctuation shadows of u v w t r
+    // the other half of the state ping-pong: the fused hydro + AB3 kernel writes
+    // the new u v w t r (and shadows) here, then the pointers swap
+    double *u2, *v2, *w2, *t2, *r2;
+    float *su2, *sv2, *sw2, *st2, *sr2;
     double *t0, *r0, *bzm;           // mixed: backgrounds of t, r; Bz-bar per plane
     int eta_on;                      // eta(z) extension active
     double *etar, *eta1;             // eta(z)/eta_0 and zeta C_k (etar - 1), nz entries
@@ -387,7 +392,9 @@ MagArgs margs(mhd_ctx* c)
 // ---------------------------------------------------------------------------
 // hydro_deriv, src/hydro_deriv.f90:5-505
 // ---------------------------------------------------------------------------
-int run_hydro_deriv(mhd_ctx* c)
+// cf: AB3 coefficients to fuse the update of u v w r t into the hydro kernel
+// (new state in the ping-pong buffers), or null for the right-hand sides only
+int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
 {
     const int wf = c->wf32;
     void** W = c->wk;
@@ -462,6 +469,27 @@ int run_hydro_deriv(mhd_ctx* c)
         TRY(spec(c, M));
     }
     HydroColArgs a = colargs(c);
+    if (cf) {
+        a.ab3 = 1;
+        a.c0 = cf[0]; a.c1 = cf[1]; a.c2 = cf[2];
+        const int hf[5] = {0, 1, 2, 3, 4};   // u v w r t
+        for (int i = 0; i < 5; ++i) {
+            a.h1[i] = level_ptr(c, hf[i], 1);
+            a.h2[i] = level_ptr(c, hf[i], 2);
+        }
+        a.h3z[0] = level_ptr(c, 3, 1);
+        a.h3z[1] = level_ptr(c, 3, 2);
+        a.h3z[2] = level_ptr(c, 3, 3);
+        const double* x64[5] = {c->u, c->v, c->w, c->r, c->t};
+        double* xn[5] = {c->u2, c->v2, c->w2, c->r2, c->t2};
+        float* sn[5] = {c->su2, c->sv2, c->sw2, c->sr2, c->st2};
+        for (int i = 0; i < 5; ++i) {
+            a.x64[i] = x64[i];
+            a.xnew[i] = xn[i];
+            a.snew[i] = sn[i];
+        }
+        if (c->su) { a.t0 = c->t0; a.r0 = c->r0; }
+    }
     return timed(c, MHD_K_HYDRO, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); });
 }
 
@@ -776,12 +804,16 @@ void ab3_coeffs(mhd_scalars_t& s, double dt, double cf[3])
     }
 }
 
-int run_ab3(mhd_ctx* c, const double cf[3])
+// f0 = 5: the hydro fields were updated by the fused hydro kernel; pol, tor
+// and the means here.  Then the history ring turns and, for f0 = 5, the state
+// ping-pong swaps.
+int run_ab3(mhd_ctx* c, const double cf[3], int f0 = 0)
 {
     AB3Args a;
     memset(&a, 0, sizeof(a));
     a.g = c->g;
     a.c0 = cf[0]; a.c1 = cf[1]; a.c2 = cf[2];
+    a.f0 = f0;
     double* vars[7] = {c->u, c->v, c->w, c->r, c->t, c->pol, c->tor};
     for (int f = 0; f < 7; ++f) {
         a.var[f] = vars[f];
@@ -801,7 +833,13 @@ int run_ab3(mhd_ctx* c, const double cf[3])
     a.nfields = a.mag ? 7 : 5;
     a.icutx = c->icutx;
     a.icuty = c->icuty;
-    TRY(timed(c, MHD_K_AB3, [&] { return kerr(launch_ab3(a, c->s)); }));
+    if (a.f0 < a.nfields || a.mag) TRY(timed(c, MHD_K_AB3, [&] { return kerr(launch_ab3(a, c->s)); }));
+    if (f0 == 5) {
+        std::swap(c->u, c->u2); std::swap(c->v, c->v2); std::swap(c->w, c->w2);
+        std::swap(c->t, c->t2); std::swap(c->r, c->r2);
+        std::swap(c->su, c->su2); std::swap(c->sv, c->sv2); std::swap(c->sw, c->sw2);
+        std::swap(c->st, c->st2); std::swap(c->sr, c->sr2);
+    }
     // shift 2->1, 3->2 (src/main.F90:448-476): relabel the ring
     const int r0 = c->ring[0];
     c->ring[0] = c->ring[1];
@@ -908,6 +946,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     bytes += 4 * fw;                                  // bx by bz jsq
     bytes += 21 * fh + 6 * vz;                        // history
     bytes += 5 * fs + 3 * vz;                         // shadows, backgrounds
+    bytes += 5 * f64 + 5 * fs;                        // state ping-pong
     c->eta_on = p->eta_c > 1.0;
     bytes += 2 * vz + (c->eta_on ? 3 * fw : 0);       // eta(z) profile, J
     bytes += W_COUNT * fw;                            // work
@@ -963,7 +1002,11 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     if (c->wf32) {
         c->su = (float*)take(fs); c->sv = (float*)take(fs); c->sw = (float*)take(fs);
         c->st = (float*)take(fs); c->sr = (float*)take(fs);
+        c->su2 = (float*)take(fs); c->sv2 = (float*)take(fs); c->sw2 = (float*)take(fs);
+        c->st2 = (float*)take(fs); c->sr2 = (float*)take(fs);
     }
+    c->u2 = (double*)take(f64); c->v2 = (double*)take(f64); c->w2 = (double*)take(f64);
+    c->t2 = (double*)take(f64); c->r2 = (double*)take(f64);
     if ((size_t)(cur - c->pool) > bytes) {
         fprintf(stderr, "mhd: pool overflow\n");
         mhd_destroy(c);
@@ -1329,12 +1372,12 @@ int mhd_step(mhd_ctx_t* c, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* l
         S.nit += 1;
         S.iter += 1;
         S.ttime += S.dt;
-        TRY(run_hydro_deriv(c));
-        if (mag) TRY(run_mag_deriv(c));
         double cf[3];
         const double dt = S.dt;
         ab3_coeffs(S, dt, cf);
-        TRY(run_ab3(c, cf));   // includes magdealias (src/main.F90:508)
+        TRY(run_hydro_deriv(c, cf));   // hydro RHS + AB3 of u v w r t into the ping-pong
+        if (mag) TRY(run_mag_deriv(c)); // reads the old u v w
+        TRY(run_ab3(c, cf, 5));        // pol, tor, means, magdealias (src/main.F90:508); swap
         S.dt2 = dt + S.dt1;
         S.dt1 = dt;
         if (S.nit % 10 == 0) TRY(run_dealias5(c));

@@ -489,36 +489,37 @@ __global__ void __launch_bounds__(NT) ab3_kernel(AB3Args av)
     const long q = (long)blockIdx.x * NT + threadIdx.x;
     if (q < a.g.n3) {
         const double c0 = a.c0, c1 = a.c1, c2 = a.c2;
-        double xn[7];
+        // every entry defined; field f is updated when f0 <= f < nfields
+        double xn[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int f = 0; f < 7; ++f) {
-            if (f >= a.nfields) break;
-            double* x = a.var[f];
-            xn[f] = x[q] + c0 * (double)((const TH*)a.hist[f][2])[q] + c1 * (double)((const TH*)a.hist[f][1])[q] +
-                    c2 * (double)((const TH*)a.hist[f][0])[q];
+            if (f >= a.f0 && f < a.nfields) {
+                const double* x = a.var[f];
+                xn[f] = x[q] + c0 * (double)((const TH*)a.hist[f][2])[q] +
+                        c1 * (double)((const TH*)a.hist[f][1])[q] + c2 * (double)((const TH*)a.hist[f][0])[q];
+            }
         }
         // src/main.F90:417-425: density floor, its history zeroed
-        if (xn[3] < 0.05) {
+        const bool hydro = a.f0 == 0;
+        if (hydro && xn[3] < 0.05) {
             xn[3] = 0.05;
             ((TH*)a.hist[3][0])[q] = (TH)0;
             ((TH*)a.hist[3][1])[q] = (TH)0;
             ((TH*)a.hist[3][2])[q] = (TH)0;
         }
+        // magdealias: pol and tor vanish on the 2/3 band
+        bool band = false;
         if (a.mag) {
             const int nx = a.g.nx, ny = a.g.ny;
             const int rem = (int)(q % a.g.nxy);
             const int jq = rem / nx, ip = rem - jq * nx;
-            if (in_band(ip, jq, nx, ny, a.icutx, a.icuty)) {
-                xn[5] = 0.0;
-                xn[6] = 0.0;
-            }
+            band = in_band(ip, jq, nx, ny, a.icutx, a.icuty);
         }
 #pragma unroll
         for (int f = 0; f < 7; ++f) {
-            if (f >= a.nfields) break;
-            a.var[f][q] = xn[f];
+            if (f >= a.f0 && f < a.nfields) a.var[f][q] = (f >= 5 && band) ? 0.0 : xn[f];
         }
-        if (a.shadow[0]) {
+        if (a.shadow[0] && hydro) {
             const int k = (int)(q / a.g.nxy);
             ((float*)a.shadow[0])[q] = (float)xn[0];
             ((float*)a.shadow[1])[q] = (float)xn[1];

@@ -67,6 +67,17 @@ struct HydroColArgs {
     const void *ry, *ty, *tyy, *uy, *uyy, *vy, *vyyc, *wy, *wyy, *ybb, *yzy, *dym, *uxy, *vxy;
     void *rdot, *tdot, *udot, *vdot, *wdot;   // newest history level (working precision)
     int zc;                             // planes per thread
+    // fused AB3 update of u v w r t (src/main.F90:413-443); ab3 = 0: RHS only.
+    // The new state goes to separate buffers, so the column walks of other
+    // chunks and the magnetic pass still read the old one.
+    int ab3;
+    double c0, c1, c2;
+    const void* h1[5];                  // history levels 1 and 2 (u v w r t order)
+    const void* h2[5];
+    void* h3z[3];                       // rdot levels 1 2 3, zeroed by the density floor
+    const double* x64[5];               // mixed: the FP64 state (the windows hold FP32 shadows)
+    double* xnew[5];                    // new FP64 state
+    float* snew[5];                     // mixed: new FP32 shadows
 };
 int launch_hydro_col(const HydroColArgs& a, int mixed, hipStream_t s);
 
@@ -101,6 +112,7 @@ int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out /*de
 struct AB3Args {
     Geom g;
     double c0, c1, c2;
+    int f0;                   // first field updated (5: pol, tor only, the hydro fields fused)
     double* var[7];           // u v w r t pol tor
     void* hist[7][3];         // [field][level 1..3] physical slots
     int h32;                  // history arrays are float
