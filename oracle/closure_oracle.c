@@ -1,0 +1,234 @@
+/*
+ * closure_oracle.c -- CPU oracle of the learned subgrid closure.
+ *
+ * TEST INFRASTRUCTURE ONLY, like mhd_oracle.c: it is the checker the HIP
+ * closure kernels are compared against and is never linked into the product.
+ * The closure has no counterpart in the reference; what it builds on is the
+ * reference's box filter of coarse-grained data (interpolate/main.F90:299-306,
+ * restated in orc_box_filter) and its field order (src/netcdf.F90: R, T, U, V,
+ * W, Bx, By, Bz).
+ *
+ * Definitions (F = the f x f x f block mean, fields r t u v w bx by bz):
+ *   R_ij = F(u_i u_j) - F(u_i) F(u_j),  M_ij = F(B_i B_j) - F(B_i) F(B_j)
+ *   tau_ij = F(r) R_ij - fmag (M_ij - delta_ij (M_xx + M_yy + M_zz) / 2)
+ *            order xx yy zz xy xz yz                         (targets 0..5)
+ *   eps = F(u x B) - F(u) x F(B)                             (targets 6..8)
+ *   q_i = F(u_i T) - F(u_i) F(T)                             (targets 9..11)
+ *
+ * Network: 216 inputs (the 8 normalised fields on the 3x3x3 neighbourhood,
+ * feature = nb * 8 + field, nb = (dz+1) 9 + (dy+1) 3 + (dx+1), x and y
+ * periodic, z clamped to the walls), two hidden layers of 64 exact-erf GELU
+ * units, 12 outputs (normalised targets).  Parameters (packed):
+ *   W1[216][64] b1[64] W2[64][64] b2[64] W3[64][12] b3[12]  (18828 values)
+ * With round != 0 the products use bf16 (round to nearest even) operands at
+ * the points where the GPU kernels round: the features, the weights, the
+ * hidden activations, dL/dy and the pre-activation gradients; sums and the
+ * biases stay wide.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mhd_oracle.h"
+
+#define NF 8
+#define NIN 216
+#define NH 64
+#define NOUT 12
+#define OFF_W1 0
+#define OFF_B1 (OFF_W1 + NIN * NH)
+#define OFF_W2 (OFF_B1 + NH)
+#define OFF_B2 (OFF_W2 + NH * NH)
+#define OFF_W3 (OFF_B2 + NH)
+#define OFF_B3 (OFF_W3 + NH * NOUT)
+#define NPARAM (OFF_B3 + NOUT)
+
+int orc_closure_nparam(void) { return NPARAM; }
+
+/* float -> bf16 -> float, round to nearest even (NaN not expected here) */
+float orc_bf16(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    u &= 0xffff0000u;
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+static double rnd(double x, int round) { return round ? (double)orc_bf16((float)x) : x; }
+
+static double gelu(double z) { return 0.5 * z * (1.0 + erf(z * 0.70710678118654752440)); }
+static double dgelu(double z)
+{
+    return 0.5 * (1.0 + erf(z * 0.70710678118654752440)) + z * exp(-0.5 * z * z) * 0.39894228040143267794;
+}
+
+int orc_closure_targets(int nx, int ny, int nz, int f, double fmag, const double* const fine[8],
+                        double* const coarse[8], double* const target[12])
+{
+    if (f < 1 || nx % f || ny % f || nz % f) return -1;
+    const int cx = nx / f, cy = ny / f, cz = nz / f;
+    const double inv = 1.0 / ((double)f * f * f);
+    for (int K = 0; K < cz; K++)
+        for (int J = 0; J < cy; J++)
+            for (int I = 0; I < cx; I++) {
+                double s[8] = {0}, uu[6] = {0}, bb[6] = {0}, uxb[3] = {0}, ut[3] = {0};
+                for (int kk = 0; kk < f; kk++)
+                    for (int jj = 0; jj < f; jj++)
+                        for (int ii = 0; ii < f; ii++) {
+                            const size_t q = ((size_t)(f * K + kk) * ny + (f * J + jj)) * nx + (f * I + ii);
+                            double v[8];
+                            for (int n = 0; n < 8; n++) {
+                                v[n] = fine[n][q];
+                                s[n] += v[n];
+                            }
+                            const double u = v[2], vv = v[3], w = v[4], bx = v[5], by = v[6], bz = v[7], t = v[1];
+                            uu[0] += u * u; uu[1] += vv * vv; uu[2] += w * w;
+                            uu[3] += u * vv; uu[4] += u * w; uu[5] += vv * w;
+                            bb[0] += bx * bx; bb[1] += by * by; bb[2] += bz * bz;
+                            bb[3] += bx * by; bb[4] += bx * bz; bb[5] += by * bz;
+                            uxb[0] += vv * bz - w * by;
+                            uxb[1] += w * bx - u * bz;
+                            uxb[2] += u * by - vv * bx;
+                            ut[0] += u * t; ut[1] += vv * t; ut[2] += w * t;
+                        }
+                double m[8];
+                for (int n = 0; n < 8; n++) m[n] = s[n] * inv;
+                const double r = m[0], T = m[1], U[3] = {m[2], m[3], m[4]}, B[3] = {m[5], m[6], m[7]};
+                static const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
+                double R[6], M[6];
+                for (int c = 0; c < 6; c++) {
+                    R[c] = uu[c] * inv - U[pi[c]] * U[pj[c]];
+                    M[c] = bb[c] * inv - B[pi[c]] * B[pj[c]];
+                }
+                const double trace = M[0] + M[1] + M[2];
+                const size_t o = ((size_t)K * cy + J) * cx + I;
+                for (int n = 0; n < 8; n++) coarse[n][o] = m[n];
+                for (int c = 0; c < 6; c++) target[c][o] = r * R[c] - fmag * (M[c] - (c < 3 ? 0.5 * trace : 0.0));
+                target[6][o] = uxb[0] * inv - (U[1] * B[2] - U[2] * B[1]);
+                target[7][o] = uxb[1] * inv - (U[2] * B[0] - U[0] * B[2]);
+                target[8][o] = uxb[2] * inv - (U[0] * B[1] - U[1] * B[0]);
+                for (int c = 0; c < 3; c++) target[9 + c][o] = ut[c] * inv - U[c] * T;
+            }
+    return 0;
+}
+
+/* normalised, optionally bf16-rounded features of one coarse cell */
+static void features(int nx, int ny, int nz, const double* const fld[8], const double norm[40], size_t q,
+                     int round, double* x)
+{
+    const int i = (int)(q % (size_t)nx), j = (int)((q / (size_t)nx) % (size_t)ny), k = (int)(q / ((size_t)nx * ny));
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int nb = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                int kk = k + dz;
+                kk = kk < 0 ? 0 : (kk > nz - 1 ? nz - 1 : kk);
+                const int jj = (j + dy + ny) % ny, ii = (i + dx + nx) % nx;
+                const size_t p = ((size_t)kk * ny + jj) * nx + ii;
+                for (int n = 0; n < NF; n++) {
+                    /* (x - mu) / sd in double, then to float, then bf16 */
+                    const float v = (float)((fld[n][p] - norm[n]) / norm[8 + n]);
+                    x[nb * NF + n] = round ? (double)orc_bf16(v) : (double)v;
+                }
+            }
+}
+
+typedef struct {
+    double z1[NH], h1[NH], z2[NH], h2[NH], y[NOUT];
+} act_t;
+
+static void forward(const double* x, const double* p, int round, act_t* a)
+{
+    for (int h = 0; h < NH; h++) {
+        double s = 0.0;
+        for (int n = 0; n < NIN; n++) s += x[n] * rnd(p[OFF_W1 + n * NH + h], round);
+        a->z1[h] = s + p[OFF_B1 + h];
+        a->h1[h] = rnd(gelu(a->z1[h]), round);
+    }
+    for (int h = 0; h < NH; h++) {
+        double s = 0.0;
+        for (int n = 0; n < NH; n++) s += a->h1[n] * rnd(p[OFF_W2 + n * NH + h], round);
+        a->z2[h] = s + p[OFF_B2 + h];
+        a->h2[h] = rnd(gelu(a->z2[h]), round);
+    }
+    for (int o = 0; o < NOUT; o++) {
+        double s = 0.0;
+        for (int n = 0; n < NH; n++) s += a->h2[n] * rnd(p[OFF_W3 + n * NOUT + o], round);
+        a->y[o] = s + p[OFF_B3 + o];
+    }
+}
+
+int orc_mlp_forward(int nx, int ny, int nz, const double* const coarse[8], const double* params,
+                    const double norm[40], int round, double* const out[12])
+{
+    const size_t n3 = (size_t)nx * ny * nz;
+    double x[NIN];
+    act_t a;
+    for (size_t q = 0; q < n3; q++) {
+        features(nx, ny, nz, coarse, norm, q, round, x);
+        forward(x, params, round, &a);
+        for (int o = 0; o < NOUT; o++) out[o][q] = a.y[o] * norm[28 + o] + norm[16 + o];
+    }
+    return 0;
+}
+
+/* Loss L = sum over the listed cells and the 12 outputs of (y - t)^2 / (12 n)
+ * with t the normalised target, and its gradient with respect to the packed
+ * parameters.  cells[c] = snapshot * (nx ny nz) + cell; coarse holds 8
+ * pointers per snapshot, target 12 per snapshot. */
+int orc_mlp_grad(int nx, int ny, int nz, const double* const* coarse, const double* const* target,
+                 const double norm[40], const double* params, int ncell, const long* cells, int round,
+                 double* loss, double* grad)
+{
+    const size_t n3 = (size_t)nx * ny * nz;
+    memset(grad, 0, sizeof(double) * NPARAM);
+    double L = 0.0;
+    const double sc = 2.0 / (12.0 * (double)ncell);
+    double x[NIN];
+    act_t a;
+    for (int c = 0; c < ncell; c++) {
+        const size_t snap = (size_t)cells[c] / n3, q = (size_t)cells[c] % n3;
+        features(nx, ny, nz, coarse + 8 * snap, norm, q, round, x);
+        forward(x, params, round, &a);
+        double dy[NOUT], dyr[NOUT];
+        for (int o = 0; o < NOUT; o++) {
+            const double t = (double)(float)((target[12 * snap + o][q] - norm[16 + o]) / norm[28 + o]);
+            const double e = a.y[o] - t;
+            L += e * e;
+            dy[o] = sc * e;
+            dyr[o] = rnd(dy[o], round);
+            grad[OFF_B3 + o] += dy[o];
+        }
+        double dz2[NH], dz2r[NH];
+        for (int h = 0; h < NH; h++) {
+            double s = 0.0;
+            for (int o = 0; o < NOUT; o++) {
+                s += dyr[o] * rnd(params[OFF_W3 + h * NOUT + o], round);
+                grad[OFF_W3 + h * NOUT + o] += a.h2[h] * dyr[o];
+            }
+            dz2[h] = s * dgelu(a.z2[h]);
+            dz2r[h] = rnd(dz2[h], round);
+            grad[OFF_B2 + h] += dz2[h];
+        }
+        double dz1[NH], dz1r[NH];
+        for (int h = 0; h < NH; h++) {
+            double s = 0.0;
+            for (int n = 0; n < NH; n++) {
+                s += dz2r[n] * rnd(params[OFF_W2 + h * NH + n], round);
+                grad[OFF_W2 + h * NH + n] += a.h1[h] * dz2r[n];
+            }
+            dz1[h] = s * dgelu(a.z1[h]);
+            dz1r[h] = rnd(dz1[h], round);
+            grad[OFF_B1 + h] += dz1[h];
+        }
+        for (int n = 0; n < NIN; n++)
+            for (int h = 0; h < NH; h++) grad[OFF_W1 + n * NH + h] += x[n] * dz1r[h];
+    }
+    *loss = L / (12.0 * (double)ncell);
+    return 0;
+}

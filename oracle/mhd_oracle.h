@@ -130,6 +130,19 @@ double orc_calcdt(orc_ctx* c);
  *    [ubar, vbar, kinenergy, magenergy(F-scaled), Nu, divbmax, divbmin]   */
 int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag);
 
+/* learned subgrid closure (closure_oracle.c): targets from fine fields,
+ * bf16-rounded MLP forward and its gradient.  norm[40] = mu[8], sd[8] of the
+ * fields, tmu[12], tsd[12] of the targets. */
+int orc_closure_nparam(void);
+float orc_bf16(float x);
+int orc_closure_targets(int nx, int ny, int nz, int f, double fmag, const double* const fine[8],
+                        double* const coarse[8], double* const target[12]);
+int orc_mlp_forward(int nx, int ny, int nz, const double* const coarse[8], const double* params,
+                    const double norm[40], int round, double* const out[12]);
+int orc_mlp_grad(int nx, int ny, int nz, const double* const* coarse, const double* const* target,
+                 const double norm[40], const double* params, int ncell, const long* cells, int round,
+                 double* loss, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

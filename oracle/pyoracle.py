@@ -61,6 +61,13 @@ def lib():
         ci = ctypes.c_int
         L.orc_expand.argtypes = [ci, ci, ci, dp, dp]
         L.orc_box_filter.argtypes = [ci, ci, ci, ci, dp, dp]
+        pp = ctypes.POINTER(dp)
+        L.orc_closure_targets.argtypes = [ci, ci, ci, ci, ctypes.c_double, pp, pp, pp]
+        L.orc_mlp_forward.argtypes = [ci, ci, ci, pp, dp, dp, ci, pp]
+        L.orc_mlp_grad.argtypes = [ci, ci, ci, pp, pp, dp, dp, ci, ctypes.POINTER(ctypes.c_long), ci,
+                                   dp, dp]
+        L.orc_bf16.restype = ctypes.c_float
+        L.orc_bf16.argtypes = [ctypes.c_float]
         L.orc_field.restype = dp
         L.orc_field.argtypes = [vp, ctypes.c_int]
         L.orc_scalar.restype = ctypes.c_double
@@ -387,3 +394,69 @@ class Oracle:
         f = 0.0 if opts & OPT_KINEMATIC else (1.0 if opts & OPT_DYNAMO else self.params["f"])
         if f != 0.0 or opts & OPT_KINEMATIC:
             self.mag_init()
+
+
+# ---- learned subgrid closure (closure_oracle.c) ------------------------------
+CLOSURE_FIELDS = ("r", "t", "u", "v", "w", "bx", "by", "bz")
+CLOSURE_TARGETS = ("tau_xx", "tau_yy", "tau_zz", "tau_xy", "tau_xz", "tau_yz",
+                   "eps_x", "eps_y", "eps_z", "q_x", "q_y", "q_z")
+N_PARAM = 18828
+
+
+def _ptrs(arrs):
+    P = ctypes.POINTER(ctypes.c_double)
+    return (P * len(arrs))(*[_p(a) for a in arrs])
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def closure_targets(fine, f, fmag):
+    """Box-filtered fields and the 12 subgrid targets of a fine state (dict of
+    (nz, ny, nx) arrays) -> (coarse dict, (12, cz, cy, cx) array)."""
+    arrs = [np.ascontiguousarray(fine[n], dtype=np.float64) for n in CLOSURE_FIELDS]
+    nz, ny, nx = arrs[0].shape
+    shp = (nz // f, ny // f, nx // f)
+    coarse = [np.zeros(shp) for _ in range(8)]
+    tgt = np.zeros((12,) + shp)
+    tl = [tgt[o] for o in range(12)]
+    e = lib().orc_closure_targets(nx, ny, nz, int(f), ctypes.c_double(fmag), _ptrs(arrs), _ptrs(coarse),
+                                  _ptrs(tl))
+    if e:
+        raise OracleError(e, "orc_closure_targets")
+    return dict(zip(CLOSURE_FIELDS, coarse)), tgt
+
+
+def mlp_forward(coarse, params, norm, round_bf16=True):
+    """Closure prediction (12, nz, ny, nx) from coarse fields."""
+    arrs = [np.ascontiguousarray(coarse[n], dtype=np.float64) for n in CLOSURE_FIELDS]
+    nz, ny, nx = arrs[0].shape
+    out = np.zeros((12, nz, ny, nx))
+    ol = [out[o] for o in range(12)]
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    nm = np.ascontiguousarray(norm, dtype=np.float64)
+    lib().orc_mlp_forward(nx, ny, nz, _ptrs(arrs), _p(p), _p(nm), int(bool(round_bf16)), _ptrs(ol))
+    return out
+
+
+def mlp_grad(snaps, params, norm, cells, round_bf16=True):
+    """Loss and gradient over cells (snapshot * n3 + cell) of snapshots given
+    as a list of (coarse dict, targets (12, ...)) pairs."""
+    arrs, tl = [], []
+    for coarse, tgt in snaps:
+        arrs += [np.ascontiguousarray(coarse[n], dtype=np.float64) for n in CLOSURE_FIELDS]
+        tl += [np.ascontiguousarray(tgt[o], dtype=np.float64) for o in range(12)]
+    nz, ny, nx = arrs[0].shape
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    nm = np.ascontiguousarray(norm, dtype=np.float64)
+    c = np.ascontiguousarray(cells, dtype=np.int64)
+    loss = ctypes.c_double(0.0)
+    g = np.zeros(N_PARAM)
+    lib().orc_mlp_grad(nx, ny, nz, _ptrs(arrs), _ptrs(tl), _p(nm), _p(p), len(c),
+                       c.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), int(bool(round_bf16)),
+                       ctypes.byref(loss), _p(g))
+    return loss.value, g
