@@ -395,3 +395,159 @@ class Ensemble:
     def __exit__(self, *exc):
         self.close()
 
+
+
+CLOSURE_FIELDS = ("r", "t", "u", "v", "w", "bx", "by", "bz")
+CLOSURE_OUTPUTS = ("tau_xx", "tau_yy", "tau_zz", "tau_xy", "tau_xz", "tau_yz",
+                   "eps_x", "eps_y", "eps_z", "q_x", "q_y", "q_z")
+
+
+def _ptr_array(arrs):
+    P = ctypes.POINTER(ctypes.c_double)
+    return (P * len(arrs))(*[_dp(a) for a in arrs])
+
+
+def closure_targets(fine, factor, fmag):
+    """Box-filtered fields and the 12 subgrid targets of a fine state (dict of
+    (nz, ny, nx) arrays r t u v w bx by bz), computed on the GPU.
+    Returns (coarse dict, (12, nz/f, ny/f, nx/f) array)."""
+    arrs = [_f64(fine[n]) for n in CLOSURE_FIELDS]
+    nz, ny, nx = arrs[0].shape
+    f = int(factor)
+    shp = (nz // f, ny // f, nx // f)
+    coarse = [np.zeros(shp) for _ in range(8)]
+    tgt = np.zeros((12,) + shp)
+    tl = [tgt[o] for o in range(12)]
+    code = _abi.load().mhd_closure_targets(nx, ny, nz, f, float(fmag), _ptr_array(arrs), _ptr_array(coarse),
+                                           _ptr_array(tl))
+    if code:
+        raise MhdError(code, "mhd_closure_targets")
+    return dict(zip(CLOSURE_FIELDS, coarse)), tgt
+
+
+class Closure:
+    """The learned subgrid closure: a 216-64-64-12 stencil MLP with bf16
+    matrix-core kernels, trained on the GPU on box-filtered fine states."""
+
+    def __init__(self, seed=0):
+        self._lib = _abi.load()
+        self._c = None
+        c = ctypes.c_void_p()
+        code = self._lib.mhd_closure_create(int(seed), ctypes.byref(c))
+        if code:
+            raise MhdError(code, "mhd_closure_create")
+        self._c = c
+        self.nparam = self._lib.mhd_closure_nparam()
+
+    @staticmethod
+    def _chk(code, what):
+        if code:
+            raise MhdError(code, what)
+
+    def close(self):
+        if self._c is not None:
+            self._lib.mhd_closure_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        self._chk(self._lib.mhd_closure_get_params(
+            self._c, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), self.nparam), "get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._chk(self._lib.mhd_closure_set_params(
+            self._c, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(p)), "set_params")
+
+    @property
+    def norm(self):
+        n = np.zeros(40)
+        self._chk(self._lib.mhd_closure_get_norm(self._c, _dp(n)), "get_norm")
+        return n
+
+    @norm.setter
+    def norm(self, n):
+        n = _f64(n)
+        self._chk(self._lib.mhd_closure_set_norm(self._c, _dp(n)), "set_norm")
+
+    def add(self, coarse, target):
+        """Append one coarse snapshot: dict of fields and (12, ...) targets."""
+        arrs = [_f64(coarse[k]) for k in CLOSURE_FIELDS]
+        tl = [_f64(target[o]) for o in range(12)]
+        cz, cy, cx = arrs[0].shape
+        self._chk(self._lib.mhd_closure_add(self._c, cx, cy, cz, _ptr_array(arrs), _ptr_array(tl)), "add")
+
+    def add_fine(self, fine, factor, fmag):
+        """Box-filter a fine state and append it; returns (coarse, targets)."""
+        coarse, tgt = closure_targets(fine, factor, fmag)
+        self.add(coarse, tgt)
+        return coarse, tgt
+
+    @property
+    def samples(self):
+        n = ctypes.c_long(0)
+        self._chk(self._lib.mhd_closure_samples(self._c, ctypes.byref(n)), "samples")
+        return n.value
+
+    def train(self, steps, batch=4096, lr=1e-3):
+        loss = ctypes.c_double(0.0)
+        self._chk(self._lib.mhd_closure_train(self._c, int(steps), int(batch), float(lr), ctypes.byref(loss)),
+                  "train")
+        return loss.value
+
+    def grad(self, cells):
+        c = np.ascontiguousarray(cells, dtype=np.int64)
+        g = np.zeros(self.nparam, np.float32)
+        loss = ctypes.c_double(0.0)
+        self._chk(self._lib.mhd_closure_grad(self._c, len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                                             ctypes.byref(loss), g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                  "grad")
+        return loss.value, g
+
+    def eval(self):
+        loss = ctypes.c_double(0.0)
+        self._chk(self._lib.mhd_closure_eval(self._c, ctypes.byref(loss)), "eval")
+        return loss.value
+
+    def predict(self, coarse):
+        arrs = [_f64(coarse[k]) for k in CLOSURE_FIELDS]
+        nz, ny, nx = arrs[0].shape
+        out = np.zeros((12, nz, ny, nx))
+        ol = [out[o] for o in range(12)]
+        self._chk(self._lib.mhd_closure_predict(self._c, nx, ny, nz, _ptr_array(arrs), _ptr_array(ol)), "predict")
+        return out
+
+    def time(self, batch=8192, reps=20):
+        """(ms per forward pass over the data set, ms per training step, cells per forward pass)"""
+        f, t = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        n = ctypes.c_long(0)
+        self._chk(self._lib.mhd_closure_time(self._c, int(batch), int(reps), ctypes.byref(f), ctypes.byref(t),
+                                             ctypes.byref(n)), "time")
+        return f.value, t.value, n.value
+
+
+def _attach_closure(self, closure):
+    """Couple a trained Closure to this (coarse) run; None detaches."""
+    self._closure = closure
+    code = self._lib.mhd_closure_attach(self._ctx, closure._c if closure is not None else None)
+    if code:
+        raise MhdError(code, "mhd_closure_attach")
+
+
+Solver.attach_closure = _attach_closure
+__all__ += ["Closure", "closure_targets", "CLOSURE_FIELDS", "CLOSURE_OUTPUTS"]

@@ -92,6 +92,7 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     dp = ctypes.POINTER(ctypes.c_double)
+    fp = ctypes.POINTER(ctypes.c_float)
     sig = {
         "mhd_ra_q_to_ck_f": (None, [d, d, d, d, d, d, d, dp, dp]),
         "mhd_strerror": (ctypes.c_char_p, [i]),
@@ -133,6 +134,22 @@ def load():
         "mhd_ensemble_destroy": (None, [vp]),
         "mhd_expand": (i, [i, i, i, dp, dp]),
         "mhd_box_filter": (i, [i, i, i, i, dp, dp]),
+        "mhd_closure_nparam": (i, []),
+        "mhd_closure_create": (i, [ctypes.c_uint, ctypes.POINTER(vp)]),
+        "mhd_closure_destroy": (None, [vp]),
+        "mhd_closure_set_params": (i, [vp, fp, i]),
+        "mhd_closure_get_params": (i, [vp, fp, i]),
+        "mhd_closure_targets": (i, [i, i, i, i, d, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp)]),
+        "mhd_closure_add": (i, [vp, i, i, i, ctypes.POINTER(dp), ctypes.POINTER(dp)]),
+        "mhd_closure_samples": (i, [vp, ctypes.POINTER(ctypes.c_long)]),
+        "mhd_closure_get_norm": (i, [vp, dp]),
+        "mhd_closure_set_norm": (i, [vp, dp]),
+        "mhd_closure_train": (i, [vp, i, i, d, dp]),
+        "mhd_closure_grad": (i, [vp, i, ctypes.POINTER(ctypes.c_long), dp, fp]),
+        "mhd_closure_eval": (i, [vp, dp]),
+        "mhd_closure_predict": (i, [vp, i, i, i, ctypes.POINTER(dp), ctypes.POINTER(dp)]),
+        "mhd_closure_time": (i, [vp, i, i, dp, dp, ctypes.POINTER(ctypes.c_long)]),
+        "mhd_closure_attach": (i, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -20,6 +20,7 @@
 #include <utility>
 #include <vector>
 
+#include "closure.hpp"
 #include "physics.hpp"
 #include "spectral.hpp"
 #include "zstencil.hpp"
@@ -74,6 +75,13 @@ struct mhd_ctx {
     double *u2, *v2, *w2, *t2, *r2;
     float *su2, *sv2, *sw2, *st2, *sr2;
     double *t0, *r0, *bzm;           // mixed: backgrounds of t, r; Bz-bar per plane
+    // learned subgrid closure: the network, its 12 fluxes, their x/y/z
+    // derivatives (working precision) and a vector of -1 (eps enters the EMF
+    // through the load's extra term)
+    mhd_closure* closure;
+    void* cl_mem;
+    void *cflux[12], *cdx[4], *cdy[4], *cdz[4];
+    double* cneg1;
     int eta_on;                      // eta(z) extension active
     double *etar, *eta1;             // eta(z)/eta_0 and zeta C_k (etar - 1), nz entries
     void *jx, *jy, *jz;              // eta(z): current density kept for the EMF
@@ -394,8 +402,15 @@ MagArgs margs(mhd_ctx* c)
 // ---------------------------------------------------------------------------
 // cf: AB3 coefficients to fuse the update of u v w r t into the hydro kernel
 // (new state in the ping-pong buffers), or null for the right-hand sides only
+int run_closure_flux(mhd_ctx* c);
+int run_closure_tend(mhd_ctx* c);
+
 int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
 {
+    if (c->closure) {
+        if (cf) return MHD_E_STATE;   // the closure's terms need the unfused update
+        TRY(run_closure_flux(c));
+    }
     const int wf = c->wf32;
     void** W = c->wk;
     const Phys& P = c->ph;
@@ -490,7 +505,50 @@ int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
         }
         if (c->su) { a.t0 = c->t0; a.r0 = c->r0; }
     }
-    return timed(c, MHD_K_HYDRO, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); });
+    TRY(timed(c, MHD_K_HYDRO, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); }));
+    return c->closure ? run_closure_tend(c) : 0;
+}
+
+// the closure's fluxes of the current state (B with its means in the mixed path)
+int run_closure_flux(mhd_ctx* c)
+{
+    ClSource src;
+    memset(&src, 0, sizeof(src));
+    const double* f64[5] = {c->r, c->t, c->u, c->v, c->w};
+    for (int q = 0; q < 5; ++q) src.p[q] = f64[q];
+    src.p[5] = c->bx;
+    src.p[6] = c->by;
+    src.p[7] = c->bz;
+    for (int q = 5; q < 8; ++q) src.f32[q] = c->wf32;
+    if (c->su) {
+        src.mean[5] = c->bxbar;
+        src.mean[6] = c->bybar;
+        src.add[7] = c->sc.bzbar;
+    }
+    const int e = closure_fluxes(c->closure, src, c->nx, c->ny, c->nz, c->cflux, c->wf32, c->s);
+    return e ? (e == MHD_E_ALLOC ? e : MHD_E_DEVICE) : 0;
+}
+
+// -div(tau)/r into udot, vdot, wdot and -div(q) into tdot (newest level)
+int run_closure_tend(mhd_ctx* c)
+{
+    const int wf = c->wf32;
+    // x: tau_xx, tau_xy, tau_xz, q_x; y: tau_xy, tau_yy, tau_yz, q_y; z: tau_xz, tau_yz, tau_zz, q_z
+    static const int ix[4] = {0, 3, 4, 9}, iy[4] = {3, 1, 5, 10}, iz[4] = {4, 5, 2, 11};
+    for (int dir = 0; dir < 2; ++dir) {
+        SpecLaunch L = spec_base(c, dir, SM_DERIV1);
+        for (int m = 0; m < 4; ++m)
+            L.job[m] = sjob(c->cflux[dir == 0 ? ix[m] : iy[m]], wf, dir == 0 ? c->cdx[m] : c->cdy[m], nullptr,
+                            MUL_D1, 0, wf);
+        L.njobs = 4;
+        TRY(spec(c, L));
+    }
+    ZLaunch Z = zbase(c);
+    for (int m = 0; m < 4; ++m) Z.job[m] = zjob(c->cflux[iz[m]], wf, c->cdz[m], wf, ZK_D1, 5);
+    Z.njobs = 4;
+    TRY(zrun(c, Z));
+    void* dots[4] = {level_ptr(c, 0, 3), level_ptr(c, 1, 3), level_ptr(c, 2, 3), level_ptr(c, 4, 3)};
+    return kerr(closure_tend(c->n3, c->cdx, c->cdy, c->cdz, wf, c->r, dots, c->h32, c->s));
 }
 
 // ---------------------------------------------------------------------------
@@ -525,6 +583,12 @@ int run_mag_deriv(mhd_ctx* c)
             L.job[0].jin = c->jx; L.job[0].jcoef = c->eta1;
             L.job[1].jin = c->jy; L.job[1].jcoef = c->eta1;
             L.job[2].jin = c->jz; L.job[2].jcoef = c->eta1;
+        } else if (c->closure) {
+            // closure: E = u x B + eps
+            for (int q = 0; q < 3; ++q) {
+                L.job[q].jin = c->cflux[6 + q];
+                L.job[q].jcoef = c->cneg1;
+            }
         }
         L.njobs = 3;
         TRY(spec(c, L));
@@ -1078,6 +1142,7 @@ void mhd_destroy(mhd_ctx_t* c)
     for (int i = 0; i < c->ev_created; ++i) (void)hipEventDestroy(c->ev[i]);
     if (c->hbuf) (void)hipHostFree(c->hbuf);
     if (c->pool) (void)hipFree(c->pool);
+    if (c->cl_mem) (void)hipFree(c->cl_mem);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c;
 }
@@ -1375,9 +1440,16 @@ int mhd_step(mhd_ctx_t* c, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* l
         double cf[3];
         const double dt = S.dt;
         ab3_coeffs(S, dt, cf);
-        TRY(run_hydro_deriv(c, cf));   // hydro RHS + AB3 of u v w r t into the ping-pong
-        if (mag) TRY(run_mag_deriv(c)); // reads the old u v w
-        TRY(run_ab3(c, cf, 5));        // pol, tor, means, magdealias (src/main.F90:508); swap
+        if (c->closure) {
+            // the closure's terms join the right-hand sides, so the update stays separate
+            TRY(run_hydro_deriv(c));
+            if (mag) TRY(run_mag_deriv(c));
+            TRY(run_ab3(c, cf));
+        } else {
+            TRY(run_hydro_deriv(c, cf));   // hydro RHS + AB3 of u v w r t into the ping-pong
+            if (mag) TRY(run_mag_deriv(c)); // reads the old u v w
+            TRY(run_ab3(c, cf, 5));        // pol, tor, means, magdealias (src/main.F90:508); swap
+        }
         S.dt2 = dt + S.dt1;
         S.dt1 = dt;
         if (S.nit % 10 == 0) TRY(run_dealias5(c));
@@ -1458,6 +1530,7 @@ int mhd_ensemble_create(mhd_ctx_t* const* boxes, int nbox, double dt, mhd_ensemb
     for (int b = 0; b < nbox; ++b) {
         if (!boxes[b]) return MHD_E_ARG;
         if (!boxes[b]->have_state) return MHD_E_STATE;
+        if (boxes[b]->closure) return MHD_E_ARG;
         for (int b2 = 0; b2 < b; ++b2)
             if (boxes[b2] == boxes[b]) return MHD_E_ARG;
     }
@@ -1651,6 +1724,35 @@ long mhd_ensemble_nodes(mhd_ensemble_t* e) { return e ? e->nodes : 0; }
 void mhd_ensemble_destroy(mhd_ensemble_t* e)
 {
     if (e) ens_free(e);
+}
+
+int mhd_closure_attach(mhd_ctx_t* c, mhd_closure_t* cl)
+{
+    TRY(check_ctx(c));
+    if (cl && c->eta_on) return MHD_E_ARG;
+    TRY(mhd_sync(c));
+    if (!cl) {
+        c->closure = nullptr;
+        return 0;
+    }
+    if (!c->cl_mem) {
+        const size_t fw = (size_t)c->n3 * (c->wf32 ? 4 : 8);
+        const size_t fa = (fw + 255) & ~(size_t)255;
+        if (hipMalloc(&c->cl_mem, 24 * fa + (size_t)c->nz * 8) != hipSuccess) {
+            c->cl_mem = nullptr;
+            return MHD_E_ALLOC;
+        }
+        char* p = (char*)c->cl_mem;
+        for (int q = 0; q < 12; ++q, p += fa) c->cflux[q] = p;
+        for (int m = 0; m < 4; ++m, p += fa) c->cdx[m] = p;
+        for (int m = 0; m < 4; ++m, p += fa) c->cdy[m] = p;
+        for (int m = 0; m < 4; ++m, p += fa) c->cdz[m] = p;
+        c->cneg1 = (double*)p;
+        std::vector<double> neg(c->nz, -1.0);
+        MHD_HIP_CHECK(hipMemcpy(c->cneg1, neg.data(), (size_t)c->nz * 8, hipMemcpyHostToDevice));
+    }
+    c->closure = cl;
+    return 0;
 }
 
 int mhd_global_averages(mhd_ctx_t* c, double out[4])

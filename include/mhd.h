@@ -234,6 +234,57 @@ int mhd_expand(int nx, int ny, int nz, const double* in, double* out);
  * reference.  MHD_E_ARG when f does not divide nx, ny and nz. */
 int mhd_box_filter(int nx, int ny, int nz, int f, const double* in, double* out);
 
+/* ---- learned subgrid closure for coarse runs (no counterpart in the
+ * reference) ------------------------------------------------------------------
+ * A stencil MLP maps the 8 coarse fields (r t u v w bx by bz) on a cell's
+ * 3x3x3 neighbourhood (216 inputs; x, y periodic, z clamped) through two
+ * 64-unit GELU layers to 12 subgrid fluxes: tau_xx yy zz xy xz yz, eps_x y z,
+ * q_x y z.  Its targets are box filters (f x f x f block means, the
+ * coarse-graining of interpolate/main.F90:299-306) of fine states:
+ *   tau_ij = F(r) [F(u_i u_j) - F(u_i) F(u_j)]
+ *            - fmag ([F(B_i B_j) - F(B_i) F(B_j)] - delta_ij (trace) / 2)
+ *   eps = F(u x B) - F(u) x F(B),   q_i = F(u_i T) - F(u_i) F(T).
+ * bf16 weights and activations, FP32 accumulation (matrix cores); FP32 master
+ * weights trained with Adam on the GPU.  Parameters are packed as
+ * W1[216][64] b1[64] W2[64][64] b2[64] W3[64][12] b3[12]; norm[40] holds the
+ * field means and deviations (8 + 8) and the target means and deviations
+ * (12 + 12) that inputs and outputs are scaled by. */
+typedef struct mhd_closure mhd_closure_t;
+int mhd_closure_nparam(void);
+int mhd_closure_create(unsigned seed, mhd_closure_t** out);
+void mhd_closure_destroy(mhd_closure_t* c);
+int mhd_closure_set_params(mhd_closure_t* c, const float* p, int n);
+int mhd_closure_get_params(mhd_closure_t* c, float* p, int n);
+/* box-filtered fields and targets of a fine state (host arrays, fields in
+ * the order r t u v w bx by bz), computed on the GPU */
+int mhd_closure_targets(int nx, int ny, int nz, int f, double fmag, const double* const fine[8],
+                        double* const coarse[8], double* const target[12]);
+/* append one coarse snapshot (all snapshots share one shape) to the data set */
+int mhd_closure_add(mhd_closure_t* c, int cx, int cy, int cz, const double* const coarse[8],
+                    const double* const target[12]);
+int mhd_closure_samples(mhd_closure_t* c, long* n);
+/* the scaling fitted to the data set (until set explicitly) */
+int mhd_closure_get_norm(mhd_closure_t* c, double norm[40]);
+int mhd_closure_set_norm(mhd_closure_t* c, const double norm[40]);
+/* `steps` Adam steps on batches of `batch` (a multiple of 32) samples drawn
+ * from the data set; *loss = the last batch's mean squared error (scaled) */
+int mhd_closure_train(mhd_closure_t* c, int steps, int batch, double lr, double* loss);
+/* loss and gradient over the listed samples (snapshot * cells + cell), no update */
+int mhd_closure_grad(mhd_closure_t* c, int ncell, const long* cells, double* loss, float* grad);
+/* mean squared error (scaled) over the whole data set */
+int mhd_closure_eval(mhd_closure_t* c, double* loss);
+/* the 12 fluxes of coarse fields (host arrays) */
+int mhd_closure_predict(mhd_closure_t* c, int nx, int ny, int nz, const double* const coarse[8],
+                        double* const out[12]);
+/* average times of one forward pass over the data set and of one training
+ * step of `batch` samples (HIP events) */
+int mhd_closure_time(mhd_closure_t* c, int batch, int reps, double* ms_fwd, double* ms_train, long* n_fwd);
+/* Attach a trained closure to a coarse run (NULL detaches): every step then
+ * adds -div(tau)/r to the momentum right-hand sides, -div(q) to T's and eps
+ * to the EMF.  The closure must outlive the attachment.  MHD_E_ARG with the
+ * eta(z) extension on. */
+int mhd_closure_attach(mhd_ctx_t* ctx, mhd_closure_t* c);
+
 /* Waits for all work of the context; returns any pending device error. */
 int mhd_sync(mhd_ctx_t* ctx);
 /* Bytes of device memory held by the context. */

@@ -262,6 +262,12 @@ struct orc_ctx {
     /* driver state, src/main.F90:84-116 */
     double dt, dt1, dt2, ttime;
     int nit, iter, first, second;
+    /* learned subgrid closure (closure_oracle.c; no counterpart in the
+     * reference): parameters, scaling and the 12 fluxes of the step */
+    double* cl_params;
+    double cl_norm[40];
+    double* cl_flux;
+    int cl_fixed;      /* fluxes given by orc_closure_fix_fluxes, not evaluated */
 };
 
 static const double third = 1.0 / 3.0; /* src/constants.f90:15 */
@@ -389,7 +395,77 @@ void orc_destroy(orc_ctx* c)
     if (!c) return;
     for (int id = 0; id < ORC_NFIELDS; id++) free(c->fld[id]);
     free(c->rowbuf); free(c->rowbuf2);
+    free(c->cl_params); free(c->cl_flux);
     free(c);
+}
+
+/* Couple a closure network to the step (params NULL detaches): every
+ * hydro_deriv then evaluates the network on the state (bf16 rounding as on
+ * the GPU) and subtracts div(tau)/r from the momentum right-hand sides and
+ * div(q) from T's (x, y spectral, z with the flag-5 stencils); mag_deriv adds
+ * eps to u x B. */
+int orc_closure_attach(orc_ctx* c, const double* params, const double norm[40])
+{
+    free(c->cl_params);
+    free(c->cl_flux);
+    c->cl_params = NULL;
+    c->cl_flux = NULL;
+    c->cl_fixed = 0;
+    if (!params) return 0;
+    const int np = orc_closure_nparam();
+    c->cl_params = (double*)malloc(sizeof(double) * (size_t)np);
+    c->cl_flux = (double*)calloc(12 * c->n3, sizeof(double));
+    if (!c->cl_params || !c->cl_flux) return -5;
+    memcpy(c->cl_params, params, sizeof(double) * (size_t)np);
+    memcpy(c->cl_norm, norm, sizeof(c->cl_norm));
+    return 0;
+}
+
+/* The same coupling with the 12 fluxes held fixed (checks the coupling on its
+ * own, with the fluxes a GPU network produced); NULL returns to no closure. */
+int orc_closure_fix_fluxes(orc_ctx* c, const double* flux)
+{
+    orc_closure_attach(c, NULL, NULL);
+    if (!flux) return 0;
+    c->cl_flux = (double*)malloc(sizeof(double) * 12 * c->n3);
+    if (!c->cl_flux) return -5;
+    memcpy(c->cl_flux, flux, sizeof(double) * 12 * c->n3);
+    c->cl_fixed = 1;
+    return 0;
+}
+
+static int closure_tend(orc_ctx* c)
+{
+    const size_t n3 = c->n3;
+    double* f[8] = {c->fld[ORC_R], c->fld[ORC_T], c->fld[ORC_U], c->fld[ORC_V],
+                    c->fld[ORC_W], c->fld[ORC_BX], c->fld[ORC_BY], c->fld[ORC_BZ]};
+    double* out[12];
+    for (int o = 0; o < 12; o++) out[o] = c->cl_flux + (size_t)o * n3;
+    if (!c->cl_fixed)
+        orc_mlp_forward(c->nx, c->ny, c->nz, (const double* const*)f, c->cl_params, c->cl_norm, 1, out);
+    double* acc = (double*)malloc(sizeof(double) * n3);
+    double* d = (double*)malloc(sizeof(double) * n3);
+    if (!acc || !d) { free(acc); free(d); return -5; }
+    /* u: tau_xx, tau_xy, tau_xz; v: tau_xy, tau_yy, tau_yz; w: tau_xz, tau_yz,
+     * tau_zz; T: q_x, q_y, q_z (flux index along x, y, z) */
+    static const int ix[4] = {0, 3, 4, 9}, iy[4] = {3, 1, 5, 10}, iz[4] = {4, 5, 2, 11};
+    double* dots[4] = {c->fld[ORC_UDOT] + 2 * n3, c->fld[ORC_VDOT] + 2 * n3, c->fld[ORC_WDOT] + 2 * n3,
+                       c->fld[ORC_TDOT] + 2 * n3};
+    const double* r = c->fld[ORC_R];
+    int e = 0;
+    for (int m = 0; m < 4; m++) {
+        orc_d1bydx(c, out[ix[m]], acc);
+        orc_d1bydy(c, out[iy[m]], d);
+        for (size_t q = 0; q < n3; q++) acc[q] = acc[q] + d[q];
+        e |= orc_d1bydz(c, out[iz[m]], d, 5);
+        for (size_t q = 0; q < n3; q++) {
+            const double div = acc[q] + d[q];
+            dots[m][q] = m < 3 ? dots[m][q] - div / r[q] : dots[m][q] - div;
+        }
+    }
+    free(acc);
+    free(d);
+    return e ? 7 : 0;
 }
 
 double* orc_field(orc_ctx* c, int id) { return (id >= 0 && id < ORC_NFIELDS) ? c->fld[id] : NULL; }
@@ -1174,6 +1250,15 @@ int orc_mag_deriv(orc_ctx* c)
             }
         }
     }
+    if (c->cl_flux) {
+        /* closure: E = u x B + eps */
+        const double* fl = c->cl_flux;
+        for (size_t q = 0; q < n3; q++) {
+            wk1[q] = wk1[q] + fl[6 * n3 + q];
+            wk2[q] = wk2[q] + fl[7 * n3 + q];
+            wk3[q] = wk3[q] + fl[8 * n3 + q];
+        }
+    }
     orc_forwardfft(c, wk1, ex);
     orc_forwardfft(c, wk2, ey);
     orc_forwardfft(c, wk3, ez);
@@ -1467,6 +1552,10 @@ int orc_hydro_deriv(orc_ctx* c)
         udot[q] = udot[q] * rr;
         vdot[q] = vdot[q] * rr;
         wdot[q] = wdot[q] * rr;
+    }
+    if (c->cl_flux) {
+        const int ec = closure_tend(c);
+        if (ec) return ec;
     }
     return e ? 7 : 0;
 }

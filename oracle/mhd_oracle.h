@@ -134,6 +134,8 @@ int orc_step(orc_ctx* c, int nsteps, double dt_fixed, int ndu06, double* diag);
  * bf16-rounded MLP forward and its gradient.  norm[40] = mu[8], sd[8] of the
  * fields, tmu[12], tsd[12] of the targets. */
 int orc_closure_nparam(void);
+int orc_closure_attach(orc_ctx* c, const double* params, const double norm[40]);
+int orc_closure_fix_fluxes(orc_ctx* c, const double* flux);
 float orc_bf16(float x);
 int orc_closure_targets(int nx, int ny, int nz, int f, double fmag, const double* const fine[8],
                         double* const coarse[8], double* const target[12]);

@@ -66,6 +66,8 @@ def lib():
         L.orc_mlp_forward.argtypes = [ci, ci, ci, pp, dp, dp, ci, pp]
         L.orc_mlp_grad.argtypes = [ci, ci, ci, pp, pp, dp, dp, ci, ctypes.POINTER(ctypes.c_long), ci,
                                    dp, dp]
+        L.orc_closure_attach.argtypes = [vp, dp, dp]
+        L.orc_closure_fix_fluxes.argtypes = [vp, dp]
         L.orc_bf16.restype = ctypes.c_float
         L.orc_bf16.argtypes = [ctypes.c_float]
         L.orc_field.restype = dp
@@ -215,6 +217,26 @@ class Oracle:
         if getattr(self, "_c", None):
             lib().orc_destroy(self._c)
             self._c = None
+
+    def attach_closure(self, params, norm):
+        """Couple a closure network (packed parameters, scaling) to the step; None detaches."""
+        if params is None:
+            lib().orc_closure_attach(self._c, None, None)
+            return
+        self._cl = (np.ascontiguousarray(params, dtype=np.float64), np.ascontiguousarray(norm, dtype=np.float64))
+        e = lib().orc_closure_attach(self._c, _p(self._cl[0]), _p(self._cl[1]))
+        if e:
+            raise OracleError(e, "orc_closure_attach")
+
+    def closure_fix_fluxes(self, flux):
+        """Couple fixed closure fluxes (12, nz, ny, nx) to the step; None detaches."""
+        if flux is None:
+            lib().orc_closure_fix_fluxes(self._c, None)
+            return
+        f = np.ascontiguousarray(flux, dtype=np.float64)
+        e = lib().orc_closure_fix_fluxes(self._c, _p(f))
+        if e:
+            raise OracleError(e, "orc_closure_fix_fluxes")
 
     # -- state access ----------------------------------------------------
     def field(self, name):
