@@ -169,6 +169,97 @@ def ensemble_bench(steps, seed=20261202):
             "capture_s": round(t_capture, 2), "identical_to_sequential": bool(same)}
 
 
+def closure_bench(nfine, seed=20261203):
+    """Learned closure: a fine run (nfine^2 x 3 nfine / 4, Ra 1e6, Q 500, T and
+    Bz noise 0.1 and 0.01, mixed precision) is box-filtered (factor 4) into
+    coarse snapshots; the stencil MLP is trained on the GPU on the earlier
+    ones.  Reported: the fraction of subgrid-flux variance it explains on a
+    later, held-out snapshot (the plain coarse method has no subgrid term and
+    scores 0); coarse runs with and without the closure from the filtered
+    held-out state against the filtered fine run (kinetic-energy spectra, Nu)
+    and the fallback decision; cost per unit of simulated time; the forward
+    and training kernels' rates."""
+    import numpy as np
+    from fug30v0kxc_amd import MHD_PREC_MIXED, Closure, Solver, closure_targets, reference_params
+    from fug30v0kxc_amd import closure_eval as ce
+    f = 4
+    fine_box = (nfine, nfine, 3 * nfine // 4)
+    coarse_box = tuple(n // f for n in fine_box)
+    p = reference_params(ra=1e6, q=500.0, t_amp=0.1, b_amp=0.01)
+    fine = Solver(*fine_box, p, precision=MHD_PREC_MIXED)
+    fine.static()
+    blk = 500
+    # spin-up with the reference's calcdt until t = 0.25
+    while fine.scalars()["ttime"] < 0.25:
+        fine.step(blk, 0.0, blk)
+    dtf = fine.scalars()["dt"]
+    snaps, nu_f = [], []
+    t0 = time.perf_counter()
+    for _ in range(6):
+        d = fine.step(blk, dtf, blk)
+        snaps.append(closure_targets(fine.get_state(), f, p["f"]))
+        nu_f.append(d["nusselt"])
+    fine.sync()
+    fine_ms_per_step = None
+    cl = Closure(seed=1)
+    for c, t in snaps[:5]:
+        cl.add(c, t)
+    t1 = time.perf_counter()
+    loss = cl.train(6000, batch=4096, lr=1e-3)
+    train_s = time.perf_counter() - t1
+    r2 = ce.explained_variance(cl.predict(snaps[5][0]), snaps[5][1])
+    fwd_ms, train_ms, nfwd = cl.time(batch=8192, reps=20)
+    flops_cell = 2 * (216 * 64 + 64 * 64 + 64 * 12)
+    # a posteriori: 4 blocks from the held-out state
+    ref = []
+    t2 = time.perf_counter()
+    for _ in range(4):
+        d = fine.step(blk, dtf, blk)
+        ref.append((closure_targets(fine.get_state(), f, p["f"])[0], d["nusselt"]))
+    fine.sync()
+    fine_ms_per_step = (time.perf_counter() - t2) / (4 * blk) * 1e3
+    fine.close()
+    runs = {}
+    for mode in ("plain", "closure"):
+        co = Solver(*coarse_box, p, precision=MHD_PREC_MIXED)
+        co.static()
+        c0 = snaps[5][0]
+        co.set_state(*[c0[n] for n in ("r", "t", "u", "v", "w", "bx", "by", "bz")])
+        if mode == "closure":
+            co.attach_closure(cl)
+        errs, nus = [], []
+        co.step(2, f * dtf, 0)          # start-up steps, untimed
+        co.sync()
+        tc = time.perf_counter()
+        nsteps = 2
+        for st_ref, _ in ref:
+            n = blk // f - (nsteps if not errs else 0)
+            d = co.step(n, f * dtf, n)
+            nsteps += n
+            errs.append(ce.spectral_error(co.get_state(), st_ref))
+            nus.append(d["nusselt"])
+        co.sync()
+        ms = (time.perf_counter() - tc) / (nsteps - 2) * 1e3
+        runs[mode] = {"spectral_error": [round(e, 4) for e in errs], "mean_spectral_error": round(float(np.mean(errs)), 4),
+                      "nusselt": [round(x, 4) for x in nus], "ms_per_step": round(ms, 4)}
+        co.close()
+    cost_fine = fine_ms_per_step * f                 # one coarse step's simulated time
+    return {
+        "fine_box": list(fine_box), "coarse_box": list(coarse_box), "filter": f, "dt_fine": dtf,
+        "training": {"snapshots": 5, "samples": cl.samples, "steps": 6000, "batch": 4096, "seconds": round(train_s, 3),
+                     "last_batch_loss": round(loss, 4)},
+        "a_priori_explained_variance": {"closure": [round(float(x), 3) for x in r2], "closure_mean": round(float(np.mean(r2)), 3),
+                                        "plain": 0.0},
+        "a_posteriori": {**runs, "fine_nusselt": [round(x[1], 4) for x in ref],
+                         "decision": ce.choose(runs["closure"]["mean_spectral_error"]),
+                         "fallback_threshold": ce.FALLBACK_SPECTRAL_ERROR},
+        "cost_ratio_fine_over_closure_coarse": round(cost_fine / runs["closure"]["ms_per_step"], 1),
+        "mlp_forward": {"cells": nfwd, "ms": round(fwd_ms, 4),
+                        "tflops": round(nfwd * flops_cell / (fwd_ms * 1e-3) / 1e12, 1), "peak_tflops_bf16": 2500.0},
+        "mlp_train_step_ms": round(train_ms, 4),
+    }
+
+
 def advance(r, total, dt):
     """Step a run on to `total` steps; returns the last diagnostics."""
     left = total - r["done"]
@@ -218,7 +309,9 @@ def main():
                          "(e.g. for a kernel profile of one mode)")
     ap.add_argument("--kernel", default="xderiv2", choices=("xderiv2", "yderiv2"),
                     help="kernel family for the roofline object")
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4"],
+    ap.add_argument("--c5-fine", type=int, default=256,
+                    help="c5 (closure) workload: fine nx = ny (nz = 3/4 of it); 512 is the 512^2 x 384 case")
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5"],
                     help="c2: 256x256x160 (default); c3: 512x512x256 with the eta(z) step; "
                          "c4: ensemble of 32 boxes in hipGraph replays")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -255,6 +348,20 @@ def main():
                "config": {"workload": "ensemble of 32 seeded boxes, mixed precision, hipGraph replays",
                           **{k: v for k, v in r.items() if k not in ("graph_mcell_steps_per_s",)}},
                "roofline": None, "cpu_baseline": None}
+        print(json.dumps(out))
+        return 0
+    if args.workload == "c5":
+        r = closure_bench(args.c5_fine)
+        out = {"metric": "closure: explained subgrid-flux variance (held out) and coarse/fine spectral error",
+               "value": r["a_priori_explained_variance"]["closure_mean"], "unit": "fraction", "n_gpus": 1,
+               "steps": None, "warmup": None, "ms_per_step": r["a_posteriori"]["closure"]["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "bf16 (FP32 accumulate); mixed-precision solver", "data": "synthetic",
+               "config": {"workload": "learned closure, fine run box-filtered by 4, Ra 1e6, Q 500", **r},
+               "roofline": {"bound": "mfma", "achieved": r["mlp_forward"]["tflops"], "peak": 2500.0,
+                            "unit": "TFLOP/s", "frac": round(r["mlp_forward"]["tflops"] / 2500.0, 4),
+                            "traffic": None},
+               "cpu_baseline": None}
         print(json.dumps(out))
         return 0
     wl = WORKLOADS[args.workload]
