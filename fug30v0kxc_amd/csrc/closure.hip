@@ -7,7 +7,7 @@
 // of coarse-grained data (interpolate/main.F90:299-306) applied to fine
 // states of this solver.
 //
-// Network: 216 -> 64 -> 64 -> 12, exact-erf GELU, bf16 weights and
+// Network: 216 -> 64 -> 64 -> 12, tanh-form GELU, bf16 weights and
 // activations, FP32 accumulation on the matrix cores
 // (v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)..+7],
 // B[8(l/16)..+7][l%16] and gets D[4(l/16)..+3][l%16]).
@@ -34,6 +34,7 @@
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 namespace mhd {
 namespace {
@@ -60,27 +61,19 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ bf16x8 frag(const __bf16* p) { return *(const bf16x8*)p; }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// tanh-form GELU: g(z) = z (1 + tanh(k (z + a z^3))) / 2, k = sqrt(2/pi), a = 0.044715;
+// tanh(u) = 1 - 2 / (1 + e^{2u}) with the hardware exp and reciprocal
+constexpr float GK = 0.7978845608028654f, GKA = 0.035677408136300125f;
+__device__ __forceinline__ float gelu_t(float z)
+{
+    const float u = z * (GK + GKA * z * z);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
+}
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + gelu_t(z)); }
 __device__ __forceinline__ float dgelu(float z)
 {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
-}
-
-// neighbour nb = (dz+1) 9 + (dy+1) 3 + (dx+1) of cell c (x, y periodic, z clamped)
-__device__ __forceinline__ long neighbour(long c, int nb, int nx, int ny, int nz)
-{
-    const long nxy = (long)nx * ny;
-    const int k = (int)(c / nxy);
-    const int rem = (int)(c - (long)k * nxy);
-    const int j = rem / nx, i = rem - j * nx;
-    const int dz = nb / 9 - 1, dy = (nb / 3) % 3 - 1, dx = nb % 3 - 1;
-    int kk = k + dz;
-    kk = kk < 0 ? 0 : (kk > nz - 1 ? nz - 1 : kk);
-    int jj = j + dy;
-    jj = jj < 0 ? jj + ny : (jj >= ny ? jj - ny : jj);
-    int ii = i + dx;
-    ii = ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii);
-    return (long)kk * nxy + (long)jj * nx + ii;
+    const float t = gelu_t(z);
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * (GK + 3.0f * GKA * z * z);
 }
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long x)
@@ -229,7 +222,41 @@ struct FwdArgs {
     double* lossp;         // per-workgroup sum of squared errors
 };
 
-__global__ void __launch_bounds__(FWD_NT) mlp_fwd_kernel(FwdArgs a)
+// the 27 neighbour offsets of a cell, from its (i, j, k) once: x and y wrap, z clamps
+struct Nbhd {
+    long base;                  // snapshot start
+    int xm, x0, xp, ym, y0, yp; // x indices; y offsets (j * nx)
+    long zm, z0, zp;            // z offsets (k * nx * ny)
+    __device__ __forceinline__ Nbhd(long c, long n3, int nx, int ny, int nz)
+    {
+        const long snap = c / n3;
+        base = snap * n3;
+        const int cc = (int)(c - base);
+        const int nxy = nx * ny;
+        const int k = cc / nxy;
+        const int rem = cc - k * nxy;
+        const int j = rem / nx, i = rem - j * nx;
+        xm = i == 0 ? nx - 1 : i - 1;
+        x0 = i;
+        xp = i == nx - 1 ? 0 : i + 1;
+        ym = (j == 0 ? ny - 1 : j - 1) * nx;
+        y0 = j * nx;
+        yp = (j == ny - 1 ? 0 : j + 1) * nx;
+        zm = (long)(k == 0 ? 0 : k - 1) * nxy;
+        z0 = (long)k * nxy;
+        zp = (long)(k == nz - 1 ? nz - 1 : k + 1) * nxy;
+    }
+    // neighbour nb = (dz+1) 9 + (dy+1) 3 + (dx+1), nb < 27
+    __device__ __forceinline__ long at(int nb) const
+    {
+        const int dz = nb / 9, r9 = nb - 9 * dz, dy = r9 / 3, dx = r9 - 3 * dy;
+        return base + (dz == 0 ? zm : (dz == 1 ? z0 : zp)) + (dy == 0 ? ym : (dy == 1 ? y0 : yp)) +
+               (dx == 0 ? xm : (dx == 1 ? x0 : xp));
+    }
+};
+
+// persistent: each workgroup stages the weights once and walks 128-cell tiles
+__global__ void __launch_bounds__(FWD_NT, 2) mlp_fwd_kernel(FwdArgs a)
 {
     __shared__ __bf16 sW1[NH * LD1];
     __shared__ __bf16 sW2[NH * LDH];
@@ -246,95 +273,114 @@ __global__ void __launch_bounds__(FWD_NT) mlp_fwd_kernel(FwdArgs a)
         *(bf16x8*)&sW3[(e >> 3) * LDH + (e & 7) * 8] = ((const bf16x8*)a.w3t)[e];
     __syncthreads();
 
-    const long base = (long)blockIdx.x * (4 * CELLS_PER_WAVE) + wave * CELLS_PER_WAVE;
     __bf16* H = sH[wave];
     const bf16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
-    f32x4 acc[2][4];
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) acc[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // layer 1: A fragment = the 8 fields of neighbour s*4 + l/16 of the row's cell
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
-        long c = base + rb * 16 + lr;
-        if (c > a.n - 1) c = a.n - 1;
-        const long snap0 = (c / a.n3) * a.n3;
-        const long cc = c - snap0;
-#pragma unroll
-        for (int s = 0; s < 7; ++s) {
-            const int nbr = s * 4 + lg;
-            const bf16x8 av = nbr < 27 ? a.grid[snap0 + neighbour(cc, nbr, a.nx, a.ny, a.nz)] : zero8;
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
-                acc[rb][nb] = mfma(av, frag(&sW1[(nb * 16 + lr) * LD1 + s * 32 + lg * 8]), acc[rb][nb]);
-        }
-    }
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const int col = nb * 16 + lr;
-            const float b = a.b1[col];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) H[(rb * 16 + lg * 4 + i) * LDH + col] = (__bf16)gelu(acc[rb][nb][i] + b);
-        }
-    __syncthreads();
-    // layer 2
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) acc[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const bf16x8 av = frag(&H[(rb * 16 + lr) * LDH + s * 32 + lg * 8]);
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
-                acc[rb][nb] = mfma(av, frag(&sW2[(nb * 16 + lr) * LDH + s * 32 + lg * 8]), acc[rb][nb]);
-        }
-    __syncthreads();
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const int col = nb * 16 + lr;
-            const float b = a.b2[col];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) H[(rb * 16 + lg * 4 + i) * LDH + col] = (__bf16)gelu(acc[rb][nb][i] + b);
-        }
-    __syncthreads();
-    // layer 3
-    f32x4 y[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-            y[rb] = mfma(frag(&H[(rb * 16 + lr) * LDH + s * 32 + lg * 8]), frag(&sW3[lr * LDH + s * 32 + lg * 8]),
-                         y[rb]);
-    const int o = lr;
+    const long ntiles = (a.n + 4 * CELLS_PER_WAVE - 1) / (4 * CELLS_PER_WAVE);
     float se = 0.f;
-    if (o < CL_NOUT) {
-        const float b = a.b3[o];
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long base = tile * (4 * CELLS_PER_WAVE) + wave * CELLS_PER_WAVE;
+        // all 14 neighbour loads of the wave's two row blocks first
+        bf16x8 av[2][7];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            long c = base + rb * 16 + lr;
+            if (c > a.n - 1) c = a.n - 1;
+            const Nbhd nbh(c, a.n3, a.nx, a.ny, a.nz);
+#pragma unroll
+            for (int s = 0; s < 7; ++s) {
+                const int nb = s * 4 + lg;
+                av[rb][s] = nb < 27 ? a.grid[nbh.at(nb)] : zero8;
+            }
+        }
+        // transposed products (W^T X^T): lane l gets units 4(l/16)..+3 of cell l%16,
+        // so each lane writes 4 consecutive units of one cell with one 8-byte store
+        f32x4 acc[2][4];
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const long c = base + rb * 16 + lg * 4 + i;
+            for (int mb = 0; mb < 4; ++mb) acc[rb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // layer 1
+#pragma unroll
+        for (int s = 0; s < 7; ++s)
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const bf16x8 w = frag(&sW1[(mb * 16 + lr) * LD1 + s * 32 + lg * 8]);
+                acc[0][mb] = mfma(w, av[0][s], acc[0][mb]);
+                acc[1][mb] = mfma(w, av[1][s], acc[1][mb]);
+            }
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const f32x4 b = *(const f32x4*)&a.b1[mb * 16 + lg * 4];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                bf16x4 h;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) h[i] = (__bf16)gelu(acc[rb][mb][i] + b[i]);
+                *(bf16x4*)&H[(rb * 16 + lr) * LDH + mb * 16 + lg * 4] = h;
+            }
+        }
+        __syncthreads();
+        // layer 2
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) acc[rb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const bf16x8 h0 = frag(&H[lr * LDH + s * 32 + lg * 8]);
+            const bf16x8 h1 = frag(&H[(16 + lr) * LDH + s * 32 + lg * 8]);
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const bf16x8 w = frag(&sW2[(mb * 16 + lr) * LDH + s * 32 + lg * 8]);
+                acc[0][mb] = mfma(w, h0, acc[0][mb]);
+                acc[1][mb] = mfma(w, h1, acc[1][mb]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const f32x4 b = *(const f32x4*)&a.b2[mb * 16 + lg * 4];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                bf16x4 h;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) h[i] = (__bf16)gelu(acc[rb][mb][i] + b[i]);
+                *(bf16x4*)&H[(rb * 16 + lr) * LDH + mb * 16 + lg * 4] = h;
+            }
+        }
+        __syncthreads();
+        // layer 3: lane l gets outputs 4(l/16)..+3 of cell l%16
+        f32x4 y[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                y[rb] = mfma(frag(&sW3[lr * LDH + s * 32 + lg * 8]), frag(&H[(rb * 16 + lr) * LDH + s * 32 + lg * 8]),
+                             y[rb]);
+        if (lg < 3) {
+            const f32x4 b = *(const f32x4*)&a.b3[lg * 4];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                const long c = base + rb * 16 + lr;
                 if (c >= a.n) continue;
-                const float yn = y[rb][i] + b;
-                if (a.out[0]) {
-                    const double v = (double)yn * a.tsd[o] + a.tmu[o];
-                    if (a.out_f32) ((float*)a.out[o])[c] = (float)v;
-                    else ((double*)a.out[o])[c] = v;
-                }
-                if (a.tn) {
-                    const long snap = c / a.n3;
-                    const float e = yn - a.tn[(snap * CL_NOUT + o) * a.n3 + (c - snap * a.n3)];
-                    se += e * e;
+                const long snap = a.tn ? c / a.n3 : 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int o = lg * 4 + i;
+                    const float yn = y[rb][i] + b[i];
+                    if (a.out[0]) {
+                        const double v = (double)yn * a.tsd[o] + a.tmu[o];
+                        if (a.out_f32) ((float*)a.out[o])[c] = (float)v;
+                        else ((double*)a.out[o])[c] = v;
+                    }
+                    if (a.tn) {
+                        const float e = yn - a.tn[(snap * CL_NOUT + o) * a.n3 + (c - snap * a.n3)];
+                        se += e * e;
+                    }
                 }
             }
+        }
+        __syncthreads();   // H is rewritten by the next tile
     }
     if (a.tn) {
 #pragma unroll
@@ -395,13 +441,11 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(TrainArgs a)
         for (int nb = 0; nb < 4; ++nb) z1[rb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) {
-        const long sid = sample_id(a, r0 + rb * 16 + lr);
-        const long snap0 = (sid / a.n3) * a.n3;
-        const long cc = sid - snap0;
+        const Nbhd nbh(sample_id(a, r0 + rb * 16 + lr), a.n3, a.nx, a.ny, a.nz);
 #pragma unroll
         for (int s = 0; s < 7; ++s) {
             const int nbr = s * 4 + lg;
-            const bf16x8 av = nbr < 27 ? a.grid[snap0 + neighbour(cc, nbr, a.nx, a.ny, a.nz)] : zero8;
+            const bf16x8 av = nbr < 27 ? a.grid[nbh.at(nbr)] : zero8;
 #pragma unroll
             for (int j = 0; j < 8; ++j) XT[(s * 32 + lg * 8 + j) * LDC + rb * 16 + lr] = av[j];
 #pragma unroll
@@ -777,9 +821,23 @@ FwdArgs fwd_args(const mhd_closure* c)
     return a;
 }
 
+// workgroups of the persistent forward grid: two per CU (LDS-bound), at most one per tile
+long fwd_grid(long n)
+{
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            ncu = 256;
+    }
+    const long tiles = (n + 4 * CELLS_PER_WAVE - 1) / (4 * CELLS_PER_WAVE);
+    return tiles < 2L * ncu ? tiles : 2L * ncu;
+}
+
 int launch_fwd(const FwdArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(mlp_fwd_kernel, dim3(nblk(a.n, 4 * CELLS_PER_WAVE)), dim3(FWD_NT), 0, s, a);
+    hipLaunchKernelGGL(mlp_fwd_kernel, dim3((unsigned)fwd_grid(a.n)), dim3(FWD_NT), 0, s, a);
     return ok(hipGetLastError()) ? 0 : MHD_E_DEVICE;
 }
 
@@ -1126,7 +1184,7 @@ int mhd_closure_eval(mhd_closure_t* c, double* loss)
     a.n3 = c->n3;
     a.n = c->n3 * c->nsnap;
     a.tn = c->tn;
-    const long nwg = (a.n + 4 * CELLS_PER_WAVE - 1) / (4 * CELLS_PER_WAVE);
+    const long nwg = fwd_grid(a.n);
     e = ensure((void**)&c->flossp, &c->fcap, nwg, sizeof(double));
     if (e) return e;
     a.lossp = c->flossp;

@@ -17,7 +17,7 @@
  *
  * Network: 216 inputs (the 8 normalised fields on the 3x3x3 neighbourhood,
  * feature = nb * 8 + field, nb = (dz+1) 9 + (dy+1) 3 + (dx+1), x and y
- * periodic, z clamped to the walls), two hidden layers of 64 exact-erf GELU
+ * periodic, z clamped to the walls), two hidden layers of 64 tanh-form GELU
  * units, 12 outputs (normalised targets).  Parameters (packed):
  *   W1[216][64] b1[64] W2[64][64] b2[64] W3[64][12] b3[12]  (18828 values)
  * With round != 0 the products use bf16 (round to nearest even) operands at
@@ -61,10 +61,14 @@ float orc_bf16(float x)
 
 static double rnd(double x, int round) { return round ? (double)orc_bf16((float)x) : x; }
 
-static double gelu(double z) { return 0.5 * z * (1.0 + erf(z * 0.70710678118654752440)); }
+/* tanh-form GELU, z (1 + tanh(k (z + a z^3))) / 2 with k = sqrt(2/pi), a = 0.044715 */
+#define GK 0.7978845608028654
+#define GKA 0.035677408136300125
+static double gelu(double z) { return 0.5 * z * (1.0 + tanh(z * (GK + GKA * z * z))); }
 static double dgelu(double z)
 {
-    return 0.5 * (1.0 + erf(z * 0.70710678118654752440)) + z * exp(-0.5 * z * z) * 0.39894228040143267794;
+    const double t = tanh(z * (GK + GKA * z * z));
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * (GK + 3.0 * GKA * z * z);
 }
 
 int orc_closure_targets(int nx, int ny, int nz, int f, double fmag, const double* const fine[8],
