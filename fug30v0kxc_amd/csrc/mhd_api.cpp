@@ -710,8 +710,10 @@ int run_mag_init(mhd_ctx* c)
                         if (!pc && comp == 2) v = seed * std::cos(tx * (double)(i + 1)) * std::cos(ty * (double)(j + 1));
                         f[((long)k * ny + j) * nx + i] = v;
                     }
-            MHD_HIP_CHECK(hipMemcpy(c->stage, f.data(), c->n3 * sizeof(double), hipMemcpyHostToDevice));
+            // on the context's stream, and finished before f and stage are reused
+            MHD_HIP_CHECK(hipMemcpyAsync(c->stage, f.data(), c->n3 * sizeof(double), hipMemcpyHostToDevice, c->s));
             TRY(kerr(launch_copy_cvt(c->stage, 0, dst[comp], c->wf32, c->n3, c->s)));
+            MHD_HIP_CHECK(hipStreamSynchronize(c->s));
         }
         if (mx) {
             // the seed has no plane means
@@ -982,16 +984,16 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->ring[0] = 0; c->ring[1] = 1; c->ring[2] = 2;
     c->sc.first = 1;
     c->ncu = ncu;
+    // launch shapes until mhd_autotune / mhd_set_tune: the autotuner's choices at
+    // 256x256x160 on MI355X (y transforms with two resident grids in FP64; 8
+    // planes per column-walk thread in FP64, 12 in the mixed mode)
     c->spec_var[0] = c->spec_var[1] = 0;
-    c->spec_mult[0] = c->spec_mult[1] = 1;
+    c->spec_mult[0] = 1;
+    c->spec_mult[1] = c->wf32 ? 1 : 2;
     {
-        // column-walk chunks: about 2048 workgroups of 256 columns, >= 8 planes each
-        const long bx = (c->nxy + 255) / 256;
-        long nch = 2048 / (bx > 0 ? bx : 1);
-        const long nin = c->nz > 8 ? c->nz - 8 : 1;   // interior planes
-        if (nch > nin / 8) nch = nin / 8;
-        if (nch < 1) nch = 1;
-        c->hzc = (int)((nin + nch - 1) / nch);
+        const int nin = c->nz > 8 ? c->nz - 8 : 1;   // interior planes
+        const int zc = c->wf32 ? 12 : 8;
+        c->hzc = zc < nin ? zc : nin;
     }
 
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return MHD_E_DEVICE; }
@@ -1021,7 +1023,9 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
         return MHD_E_ALLOC;
     }
     c->pool_bytes = bytes;
-    if (hipMemsetAsync(c->pool, 0, bytes, c->s) != hipSuccess) {
+    // the fill runs on the context's non-blocking stream: it must finish before
+    // the tables below are copied in (plain hipMemcpy is not ordered with it)
+    if (hipMemsetAsync(c->pool, 0, bytes, c->s) != hipSuccess || hipStreamSynchronize(c->s) != hipSuccess) {
         mhd_destroy(c);
         return MHD_E_DEVICE;
     }

@@ -131,3 +131,23 @@ def test_mixed_rejects_setting_b_by_field():
     with pytest.raises(MhdError):
         s.set_field("bx", np.zeros((8, 16, 16)))
     s.close()
+
+
+def test_mixed_large_box_keeps_fp64_backgrounds():
+    """A large box (a 3 GB pool) in the mixed mode: its FP64 backgrounds T0(z),
+    rho0(z) must survive the pool's initial fill, or T is carried as full FP32
+    values and drifts from the FP64 run by ~1e-7 within a few steps instead of
+    ~1e-10."""
+    import fug30v0kxc_amd as fa
+    print("[mixed] 256x256x160 backgrounds", flush=True)
+    p = reference_params(ra=1e6, q=1000.0, t_amp=1e-3, b_amp=1e-3)
+    a = fa.Solver(256, 256, 160, p, precision=fa.MHD_PREC_FP64)
+    a.static()
+    b = fa.Solver(256, 256, 160, p, precision=fa.MHD_PREC_MIXED)
+    b.static()
+    a.step(10, 5e-5, 0)
+    b.step(10, 5e-5, 0)
+    sa, sb = a.get_state(), b.get_state()
+    for n in ("t", "r"):
+        e = np.abs(sa[n] - sb[n]).max() / np.abs(sa[n]).max()
+        assert e < 1e-9, (n, e)
