@@ -74,6 +74,7 @@ struct mhd_ctx {
     // the new u v w t r (and shadows) here, then the pointers swap
     double *u2, *v2, *w2, *t2, *r2;
     float *su2, *sv2, *sw2, *st2, *sr2;
+    double *pol2, *tor2;             // and of pol, tor (the update fused into mag_rhs)
     double *t0, *r0, *bzm;           // mixed: backgrounds of t, r; Bz-bar per plane
     // learned subgrid closure: the network, its 12 fluxes, their x/y/z
     // derivatives (working precision) and a vector of -1 (eps enters the EMF
@@ -554,7 +555,9 @@ int run_closure_tend(mhd_ctx* c)
 // ---------------------------------------------------------------------------
 // mag_deriv, src/magnetic.f90:285-487
 // ---------------------------------------------------------------------------
-int run_mag_deriv(mhd_ctx* c)
+// cf: AB3 coefficients to fuse the update of pol, tor into mag_rhs (new
+// values in the ping-pong buffers), or null for the right-hand sides only
+int run_mag_deriv(mhd_ctx* c, const double* cf = nullptr)
 {
     const int wf = c->wf32;
     MagArgs m = margs(c);
@@ -606,6 +609,14 @@ int run_mag_deriv(mhd_ctx* c)
         Z.job[0] = zjob(m.d, wf, m.dz, wf, ZK_D1, 5);  // 419
         Z.njobs = 1;
         TRY(zrun(c, Z));
+    }
+    if (cf) {
+        m.ab3 = 1;
+        m.c0 = cf[0]; m.c1 = cf[1]; m.c2 = cf[2];
+        m.ph1 = level_ptr(c, 5, 1); m.ph2 = level_ptr(c, 5, 2);
+        m.th1 = level_ptr(c, 6, 1); m.th2 = level_ptr(c, 6, 2);
+        m.polnew = c->pol2;
+        m.tornew = c->tor2;
     }
     return kerr(launch_mag_rhs(m, c->s));
 }
@@ -871,8 +882,8 @@ void ab3_coeffs(mhd_scalars_t& s, double dt, double cf[3])
 }
 
 // f0 = 5: the hydro fields were updated by the fused hydro kernel; pol, tor
-// and the means here.  Then the history ring turns and, for f0 = 5, the state
-// ping-pong swaps.
+// and the means here.  f0 = 7: pol and tor too (fused into mag_rhs); the means
+// here.  Then the history ring turns and the state ping-pong swaps.
 int run_ab3(mhd_ctx* c, const double cf[3], int f0 = 0)
 {
     AB3Args a;
@@ -900,11 +911,15 @@ int run_ab3(mhd_ctx* c, const double cf[3], int f0 = 0)
     a.icutx = c->icutx;
     a.icuty = c->icuty;
     if (a.f0 < a.nfields || a.mag) TRY(timed(c, MHD_K_AB3, [&] { return kerr(launch_ab3(a, c->s)); }));
-    if (f0 == 5) {
+    if (f0 >= 5) {
         std::swap(c->u, c->u2); std::swap(c->v, c->v2); std::swap(c->w, c->w2);
         std::swap(c->t, c->t2); std::swap(c->r, c->r2);
         std::swap(c->su, c->su2); std::swap(c->sv, c->sv2); std::swap(c->sw, c->sw2);
         std::swap(c->st, c->st2); std::swap(c->sr, c->sr2);
+    }
+    if (f0 >= 7) {
+        std::swap(c->pol, c->pol2);
+        std::swap(c->tor, c->tor2);
     }
     // shift 2->1, 3->2 (src/main.F90:448-476): relabel the ring
     const int r0 = c->ring[0];
@@ -1012,7 +1027,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     bytes += 4 * fw;                                  // bx by bz jsq
     bytes += 21 * fh + 6 * vz;                        // history
     bytes += 5 * fs + 3 * vz;                         // shadows, backgrounds
-    bytes += 5 * f64 + 5 * fs;                        // state ping-pong
+    bytes += 7 * f64 + 5 * fs;                        // state ping-pong (u v w t r pol tor)
     c->eta_on = p->eta_c > 1.0;
     bytes += 2 * vz + (c->eta_on ? 3 * fw : 0);       // eta(z) profile, J
     bytes += W_COUNT * fw;                            // work
@@ -1075,6 +1090,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     }
     c->u2 = (double*)take(f64); c->v2 = (double*)take(f64); c->w2 = (double*)take(f64);
     c->t2 = (double*)take(f64); c->r2 = (double*)take(f64);
+    c->pol2 = (double*)take(f64); c->tor2 = (double*)take(f64);
     if ((size_t)(cur - c->pool) > bytes) {
         fprintf(stderr, "mhd: pool overflow\n");
         mhd_destroy(c);
@@ -1450,9 +1466,9 @@ int mhd_step(mhd_ctx_t* c, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* l
             if (mag) TRY(run_mag_deriv(c));
             TRY(run_ab3(c, cf));
         } else {
-            TRY(run_hydro_deriv(c, cf));   // hydro RHS + AB3 of u v w r t into the ping-pong
-            if (mag) TRY(run_mag_deriv(c)); // reads the old u v w
-            TRY(run_ab3(c, cf, 5));        // pol, tor, means, magdealias (src/main.F90:508); swap
+            TRY(run_hydro_deriv(c, cf));       // hydro RHS + AB3 of u v w r t into the ping-pong
+            if (mag) TRY(run_mag_deriv(c, cf)); // reads the old u v w; + AB3 of pol, tor, magdealias
+            TRY(run_ab3(c, cf, mag ? 7 : 5));  // the means (src/main.F90:508); swaps
         }
         S.dt2 = dt + S.dt1;
         S.dt1 = dt;

@@ -146,6 +146,17 @@ int launch_z(const ZLaunch& L, hipStream_t s)
 // ---------------------------------------------------------------------------
 // magnetic part on the double-halfcomplex layout (src/magnetic.f90)
 // ---------------------------------------------------------------------------
+// Workgroups are dealt to the 8 XCDs round-robin.  Remapping block b to tile
+// (b % 8) * (nb / 8) + b / 8 lets each XCD walk a contiguous eighth of the tiles
+// (whole planes), so the index-swap kernels find a plane's partner rows in
+// their own L2.  The tail beyond 8 * (nb / 8) keeps the identity.
+__device__ __forceinline__ long xcd_tile(long b, long nb)
+{
+    const long per = nb / 8;
+    if (b >= 8 * per) return b;
+    return (b % 8) * per + b / 8;
+}
+
 // spectral d/dx at (p,q): index swap along x, src/magnetic.f90:141-150
 template <typename TA>
 __device__ __forceinline__ double sdx(const TA* a, long base, int p, int q, int nx, const double* akx1)
@@ -171,7 +182,7 @@ __device__ __forceinline__ bool in_band(int p, int q, int nx, int ny, int icx, i
 template <typename TW>
 __global__ void __launch_bounds__(NT) mag_spec1_kernel(MagArgs a)
 {
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    const long q = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
     if (q >= a.g.n3) return;
     const int nx = a.g.nx, ny = a.g.ny;
     const int k = (int)(q / a.g.nxy);
@@ -233,6 +244,7 @@ __global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
     const int jq = (int)(col / nx), ip = (int)(col - (long)jq * nx);
     const double k2 = a.akx2[ip] + a.aky2[jq];
     const bool dc = (ip == 0 && jq == 0);
+    const bool band = in_band(ip, jq, nx, a.g.ny, a.icutx, a.icuty);
     const double* pol = a.pol;
     const double* tor = a.tor;
     const double zck = a.p.zck, s = a.p.sc1;
@@ -270,8 +282,23 @@ __global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
         }
         const double il_c = dc ? 0.0 : (double)((const TW*)a.cc)[q] / k2;
         const double il_d = dc ? 0.0 : (double)((const TW*)a.dz)[q] / k2;
-        ((TW*)a.poldot)[q] = (TW)(zck * p0 * k2 + zck * pzz + il_c);
-        ((TW*)a.tordot)[q] = (TW)(zck * t0 * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q]);
+        const TW pd = (TW)(zck * p0 * k2 + zck * pzz + il_c);
+        const TW td = (TW)(zck * t0 * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q]);
+        ((TW*)a.poldot)[q] = pd;
+        ((TW*)a.tordot)[q] = td;
+        if (a.ab3) {
+            // x + c0 f3 + c1 f2 + c2 f1, then the 2/3 band of pol and tor cleared
+            double pn = p0 + a.c0 * (double)pd + a.c1 * (double)((const TW*)a.ph2)[q] +
+                        a.c2 * (double)((const TW*)a.ph1)[q];
+            double tn = t0 + a.c0 * (double)td + a.c1 * (double)((const TW*)a.th2)[q] +
+                        a.c2 * (double)((const TW*)a.th1)[q];
+            if (band) {
+                pn = 0.0;
+                tn = 0.0;
+            }
+            a.polnew[q] = pn;
+            a.tornew[q] = tn;
+        }
     }
 }
 
@@ -279,7 +306,7 @@ __global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
 template <typename TW>
 __global__ void __launch_bounds__(NT) bandj_spec_kernel(MagArgs a)
 {
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    const long q = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
     if (q >= a.g.n3) return;
     const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
     const int k = (int)(q / nxy);
@@ -379,7 +406,7 @@ __global__ void __launch_bounds__(NT) jsq_kernel(MagArgs av)
 template <typename TW>
 __global__ void __launch_bounds__(NT) mag_init_spec_kernel(MagArgs a)
 {
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    const long q = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
     if (q >= a.g.n3) return;
     const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
     const int k = (int)(q / nxy);
@@ -538,7 +565,8 @@ __global__ void __launch_bounds__(NT) ab3_kernel(AB3Args av)
 
 int launch_ab3(const AB3Args& a, hipStream_t s)
 {
-    long n = a.g.n3 > a.g.nz ? a.g.n3 : a.g.nz;
+    // with no field to update (f0 = 7) only the nz means are left
+    long n = a.f0 < a.nfields ? (a.g.n3 > a.g.nz ? a.g.n3 : a.g.nz) : a.g.nz;
     if (a.h32) hipLaunchKernelGGL(ab3_kernel<float>, dim3(nblk(n)), dim3(NT), 0, s, a);
     else hipLaunchKernelGGL(ab3_kernel<double>, dim3(nblk(n)), dim3(NT), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -22;

@@ -100,6 +100,13 @@ struct MagArgs {
     void *jx, *jy, *jz;           // eta(z): J kept for the EMF, or null
     int wf32;
     int zc;                       // planes per thread of the column-walking kernels
+    // mag_rhs with the AB3 update of pol, tor fused (src/main.F90:413-443) and
+    // magdealias; the new values go to polnew, tornew (other threads' z-windows
+    // still read the old ones); ab3 = 0: right-hand sides only
+    int ab3;
+    double c0, c1, c2;
+    const void *ph1, *ph2, *th1, *th2;   // history levels 1 and 2 of pol, tor
+    double *polnew, *tornew;
 };
 int launch_mag_spec1(const MagArgs& a, hipStream_t s);   // D, C from E-hat
 int launch_mean_emf(const MagArgs& a, hipStream_t s);    // mean-field RHS
@@ -112,7 +119,7 @@ int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out /*de
 struct AB3Args {
     Geom g;
     double c0, c1, c2;
-    int f0;                   // first field updated (5: pol, tor only, the hydro fields fused)
+    int f0;                   // first field updated (5: pol, tor; 7: the means only)
     double* var[7];           // u v w r t pol tor
     void* hist[7][3];         // [field][level 1..3] physical slots
     int h32;                  // history arrays are float
