@@ -58,6 +58,7 @@ struct SpecLaunch {
     int grid_mult;    // ... times this (>= 1; 0 = 1), a tunable launch shape
     int variant;      // radix plan: 0 the default, 1 the length's alternative (if any)
     int max_groups;   // optional further cap on the grid; 0 = none
+    int group_major;  // set by the launcher: items ordered line group first (see spec_kernel)
 };
 
 // Returns 0 or a negative code (-20: unsupported length).
