@@ -302,43 +302,69 @@ __global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
     }
 }
 
-// spectral B and J pieces of bandj, src/magnetic.f90:519-617
+// spectral B and J pieces of bandj, src/magnetic.f90:519-617.  The index swaps
+// pair the points (j, i), (j, nx-i), (ny-j, i), (ny-j, nx-i) of a plane and
+// nothing else: one thread takes such a quad (j <= ny/2, i <= nx/2; fewer
+// points on the lines i = 0, nx/2 and j = 0, ny/2, which are their own
+// mirrors), reads pol, tor and d pol/dz once at each of its points and writes
+// all six outputs there, with the arithmetic of the per-point formulas.
 template <typename TW>
 __global__ void __launch_bounds__(NT) bandj_spec_kernel(MagArgs a)
 {
-    const long q = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-    if (q >= a.g.n3) return;
     const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
-    const int k = (int)(q / nxy);
+    const int hx = nx / 2 + 1, hy = ny / 2 + 1;
+    const long perk = (long)hx * hy;
+    const long t = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+    if (t >= perk * a.g.nz) return;
+    const int k = (int)(t / perk);
+    const int rem = (int)(t - (long)k * perk);
+    const int jj = rem / hx, ii = rem - jj * hx;
     const long base = (long)k * nxy;
-    const int rem = (int)(q - base);
-    const int jq = rem / nx, ip = rem - jq * nx;
+    const bool xm = !(ii == 0 || ii == nx / 2), ym = !(jj == 0 || jj == ny / 2);
+    const int col[2] = {ii, nx - ii}, row[2] = {jj, ny - jj};
     const TW* pz = (const TW*)a.polz;
-    const double* pol = a.pol;
-    const double* tor = a.tor;
-    const double bxh = sdx(pz, base, ip, jq, nx, a.akx1) + sdy(tor, base, ip, jq, nx, ny, a.aky1);
-    const double byh = sdy(pz, base, ip, jq, nx, ny, a.aky1) - sdx(tor, base, ip, jq, nx, a.akx1);
-    const double k2 = a.akx2[ip] + a.aky2[jq];
-    const double bzh = -1.0 * pol[q] * k2;
-    const double jzh = -1.0 * tor[q] * k2;
-    // d/dy and d/dx of bz-hat = -pol*(akx2+aky2) at the partner positions
-    double dyb = 0.0, dxb = 0.0;
-    if (!(jq == 0 || jq == ny / 2)) {
-        const int qq = ny - jq;
-        const double v = (-1.0 * pol[base + (long)qq * nx + ip] * (a.akx2[ip] + a.aky2[qq])) * a.aky1[jq];
-        dyb = (jq < ny / 2) ? -v : v;
-    }
-    if (!(ip == 0 || ip == nx / 2)) {
-        const int pp = nx - ip;
-        const double v = (-1.0 * pol[base + (long)jq * nx + pp] * (a.akx2[pp] + a.aky2[jq])) * a.akx1[ip];
-        dxb = (ip < nx / 2) ? -v : v;
-    }
-    ((TW*)a.bxh)[q] = (TW)bxh;
-    ((TW*)a.byh)[q] = (TW)byh;
-    ((TW*)a.bzh)[q] = (TW)bzh;
-    ((TW*)a.jzh)[q] = (TW)jzh;
-    ((TW*)a.dybzh)[q] = (TW)dyb;
-    ((TW*)a.dxbzh)[q] = (TW)dxb;
+    double P[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, T[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, Z[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+        for (int sx = 0; sx < 2; ++sx) {
+            if ((sy && !ym) || (sx && !xm)) continue;
+            const long q = base + (long)row[sy] * nx + col[sx];
+            P[sy][sx] = a.pol[q];
+            T[sy][sx] = a.tor[q];
+            Z[sy][sx] = (double)pz[q];
+        }
+#pragma unroll
+    for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+        for (int sx = 0; sx < 2; ++sx) {
+            if ((sy && !ym) || (sx && !xm)) continue;
+            const int p = col[sx], q = row[sy];
+            const double k2 = a.akx2[p] + a.aky2[q];
+            // d/dx and d/dy by index swap (src/magnetic.f90:141-164)
+            double dxz = 0.0, dxt = 0.0, dxb = 0.0, dyz = 0.0, dyt = 0.0, dyb = 0.0;
+            if (xm) {
+                const int pp = col[1 - sx];
+                const double sg = (p < nx / 2) ? -1.0 : 1.0;
+                dxz = sg * (Z[sy][1 - sx] * a.akx1[p]);
+                dxt = sg * (T[sy][1 - sx] * a.akx1[p]);
+                dxb = sg * ((-1.0 * P[sy][1 - sx] * (a.akx2[pp] + a.aky2[q])) * a.akx1[p]);
+            }
+            if (ym) {
+                const int qq = row[1 - sy];
+                const double sg = (q < ny / 2) ? -1.0 : 1.0;
+                dyz = sg * (Z[1 - sy][sx] * a.aky1[q]);
+                dyt = sg * (T[1 - sy][sx] * a.aky1[q]);
+                dyb = sg * ((-1.0 * P[1 - sy][sx] * (a.akx2[p] + a.aky2[qq])) * a.aky1[q]);
+            }
+            const long o = base + (long)q * nx + p;
+            ((TW*)a.bxh)[o] = (TW)(dxz + dyt);
+            ((TW*)a.byh)[o] = (TW)(dyz - dxt);
+            ((TW*)a.bzh)[o] = (TW)(-1.0 * P[sy][sx] * k2);
+            ((TW*)a.jzh)[o] = (TW)(-1.0 * T[sy][sx] * k2);
+            ((TW*)a.dybzh)[o] = (TW)dyb;
+            ((TW*)a.dxbzh)[o] = (TW)dxb;
+        }
 }
 
 // J^2, src/magnetic.f90:630-645 (dybzh/dxbzh/jzh hold physical-space fields here)
@@ -489,7 +515,8 @@ int launch_mag_rhs(const MagArgs& a, hipStream_t s)
 }
 int launch_bandj_spec(const MagArgs& a, hipStream_t s)
 {
-    MHD_LAUNCH_W(bandj_spec_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    const long quads = (long)(a.g.nx / 2 + 1) * (a.g.ny / 2 + 1) * a.g.nz;
+    MHD_LAUNCH_W(bandj_spec_kernel, dim3(nblk(quads)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_jsq(const MagArgs& a, hipStream_t s)
