@@ -179,23 +179,55 @@ __device__ __forceinline__ bool in_band(int p, int q, int nx, int ny, int icx, i
     return (p >= nx / 2 - icx && p <= nx / 2 + icx) || (q >= ny / 2 - icy && q <= ny / 2 + icy);
 }
 
+// src/magnetic.f90:362-407: cc = dEx/dy - dEy/dx, d = dEx/dx + dEy/dy by index
+// swaps, one thread per mirror quad as in bandj_spec_kernel
 template <typename TW>
 __global__ void __launch_bounds__(NT) mag_spec1_kernel(MagArgs a)
 {
-    const long q = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-    if (q >= a.g.n3) return;
-    const int nx = a.g.nx, ny = a.g.ny;
-    const int k = (int)(q / a.g.nxy);
-    const long base = (long)k * a.g.nxy;
-    const int rem = (int)(q - base);
-    const int jq = rem / nx, ip = rem - jq * nx;
+    const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
+    const int hx = nx / 2 + 1, hy = ny / 2 + 1;
+    const long perk = (long)hx * hy;
+    const long t = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+    if (t >= perk * a.g.nz) return;
+    const int k = (int)(t / perk);
+    const int rem = (int)(t - (long)k * perk);
+    const int jj = rem / hx, ii = rem - jj * hx;
+    const long base = (long)k * nxy;
+    const bool xm = !(ii == 0 || ii == nx / 2), ym = !(jj == 0 || jj == ny / 2);
+    const int col[2] = {ii, nx - ii}, row[2] = {jj, ny - jj};
     const TW* ex = (const TW*)a.exh;
     const TW* ey = (const TW*)a.eyh;
-    // 362-407: wk1 = dEx/dx + dEy/dy, wk2 = dEx/dy - dEy/dx
-    const double dxex = sdx(ex, base, ip, jq, nx, a.akx1), dxey = sdx(ey, base, ip, jq, nx, a.akx1);
-    const double dyex = sdy(ex, base, ip, jq, nx, ny, a.aky1), dyey = sdy(ey, base, ip, jq, nx, ny, a.aky1);
-    ((TW*)a.cc)[q] = (TW)(dyex - dxey);
-    ((TW*)a.d)[q] = (TW)(dxex + dyey);
+    double X[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, Y[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+        for (int sx = 0; sx < 2; ++sx) {
+            if ((sy && !ym) || (sx && !xm)) continue;
+            const long q = base + (long)row[sy] * nx + col[sx];
+            X[sy][sx] = (double)ex[q];
+            Y[sy][sx] = (double)ey[q];
+        }
+#pragma unroll
+    for (int sy = 0; sy < 2; ++sy)
+#pragma unroll
+        for (int sx = 0; sx < 2; ++sx) {
+            if ((sy && !ym) || (sx && !xm)) continue;
+            const int p = col[sx], q = row[sy];
+            double dxex = 0.0, dxey = 0.0, dyex = 0.0, dyey = 0.0;
+            if (xm) {
+                const double sg = (p < nx / 2) ? -1.0 : 1.0;
+                dxex = sg * (X[sy][1 - sx] * a.akx1[p]);
+                dxey = sg * (Y[sy][1 - sx] * a.akx1[p]);
+            }
+            if (ym) {
+                const double sg = (q < ny / 2) ? -1.0 : 1.0;
+                dyex = sg * (X[1 - sy][sx] * a.aky1[q]);
+                dyey = sg * (Y[1 - sy][sx] * a.aky1[q]);
+            }
+            const long o = base + (long)q * nx + p;
+            ((TW*)a.cc)[o] = (TW)(dyex - dxey);
+            ((TW*)a.d)[o] = (TW)(dxex + dyey);
+        }
 }
 
 // mean-field equations, src/magnetic.f90:328-352 (one workgroup)
@@ -499,7 +531,8 @@ __global__ void bzbar_kernel(MagArgs a, int wf32, double* out)
 
 int launch_mag_spec1(const MagArgs& a, hipStream_t s)
 {
-    MHD_LAUNCH_W(mag_spec1_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    const long quads = (long)(a.g.nx / 2 + 1) * (a.g.ny / 2 + 1) * a.g.nz;
+    MHD_LAUNCH_W(mag_spec1_kernel, dim3(nblk(quads)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_mean_emf(const MagArgs& a, hipStream_t s)
