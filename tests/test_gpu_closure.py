@@ -270,3 +270,34 @@ def test_closure_gain_on_fine_run_data():
     print(f"[closure] spectral error against the filtered fine run {errs}", flush=True)
     assert errs["closure"] < 0.25 and abs(errs["closure"] - errs["plain"]) < 0.1
     assert ce.choose(errs["closure"]) == ("closure" if errs["closure"] <= 0.1 else "plain")
+
+
+def test_closure_refusals():
+    """The closure needs the unfused update and the EMF's extra term: it is
+    refused with eta(z) on and inside an ensemble; bad arguments are errors."""
+    import fug30v0kxc_amd as fa
+    print("[closure] refusals", flush=True)
+    cl = fa.Closure(seed=0)
+    p = dict(reference_params(t_amp=1e-3), eta_c=10.0, eta_z0=0.5, eta_w=0.05)
+    s = fa.Solver(16, 16, 12, p)
+    s.static()
+    with pytest.raises(fa.MhdError) as e:
+        s.attach_closure(cl)
+    assert e.value.code == -10
+    q = fa.Solver(16, 16, 12, reference_params(t_amp=1e-3))
+    q.static()
+    q.attach_closure(cl)
+    with pytest.raises(fa.MhdError):
+        fa.Ensemble([q], 1e-4)
+    q.attach_closure(None)
+    with fa.Ensemble([q], 1e-4) as ens:
+        ens.step(ens.block)
+    with pytest.raises(fa.MhdError):
+        cl.train(10, batch=100)          # batch not a multiple of 32
+    with pytest.raises(fa.MhdError):
+        cl.train(10, batch=64)           # no data yet
+    with pytest.raises(fa.MhdError):
+        cl.params = np.zeros(10, np.float32)
+    bad = np.concatenate([np.zeros(8), np.zeros(8), np.zeros(12), np.ones(12)])
+    with pytest.raises(fa.MhdError):
+        cl.norm = bad                    # zero deviations
