@@ -238,9 +238,20 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
     const int icut = L.icut;
     __syncthreads();
 
+    // Column groups narrower than a 128-byte line (y direction, 2 B elements of
+    // sizeof(TC) < 128 bytes: the 512-point lengths) share each line with the
+    // next group.  Within every 16 items, item c (< 8) takes group 2c and item
+    // c + 8 group 2c + 1: workgroups w and w + 8 run on the same XCD (they are
+    // dealt round-robin over 8) at the same time, so a line is fetched once.
+    constexpr bool half_line = DIR == 1 && 2 * B * (int)sizeof(TC) < 128;
+    const bool pair = half_line && !gmaj && (G % 16 == 0) && (ngrp % 16 == 0);
     // (job, line group) of an item; false for the padding of the last round
     auto decode = [&](long item, int& jb, long& grp) -> bool {
         if (!gmaj) {
+            if (pair) {
+                const long c = item & 15;
+                item = item - c + (c < 8 ? 2 * c : 2 * (c - 8) + 1);
+            }
             jb = (int)(item / ngrp);
             grp = item - (long)jb * ngrp;
             return true;

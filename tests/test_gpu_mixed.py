@@ -84,8 +84,8 @@ def test_mixed_mag_rhs(pair):
         assert e < TOL, n
 
 
-@pytest.mark.parametrize("box,steps", [((16, 16, 12), 40), ((64, 64, 48), 30)],
-                         ids=["16x16x12-40", "64x64x48-30"])
+@pytest.mark.parametrize("box,steps", [((16, 16, 12), 40), ((64, 64, 48), 30), ((512, 512, 8), 10)],
+                         ids=["16x16x12-40", "64x64x48-30", "512x512x8-10"])
 def test_mixed_steps_against_fp64_oracle(box, steps):
     print(f"[mixed] steps {box} x {steps}", flush=True)
     p = reference_params(t_amp=1e-3, b_amp=1e-3)

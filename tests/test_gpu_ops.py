@@ -10,7 +10,7 @@ from tests.conftest import reference_params
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(16, 12, 8), (64, 64, 48), (30, 30, 12), (96, 64, 10), (256, 256, 6)]
+SIZES = [(16, 12, 8), (64, 64, 48), (30, 30, 12), (96, 64, 10), (256, 256, 6), (512, 512, 6)]
 
 
 def rel(a, b):
@@ -107,3 +107,28 @@ def test_create_error_codes():
         with pytest.raises(MhdError) as e:
             Solver(*dims, p)
         assert e.value.code == code
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_512_transforms_both_radix_plans(variant):
+    """512-point lines: column groups narrower than a cache line (paired onto
+    one XCD) and both radix plans (8x8x8 is the alternative)."""
+    from fug30v0kxc_amd import Solver
+    nx, ny, nz = 512, 512, 6
+    print(f"[ops] 512 variant {variant}", flush=True)
+    p = reference_params()
+    s = Solver(nx, ny, nz, p)
+    s.set_tune(variant_x=variant, variant_y=variant, mult_x=1, mult_y=2, zc=8)
+    o = Oracle(nx, ny, nz, p)
+    f = field(s.shape, 7)
+    assert rel(s.d1bydy(f), o.d1bydy(f)) < 1e-12
+    a1, a2 = s.d2bydy(f)
+    b1, b2 = o.d2bydy(f)
+    assert rel(a1, b1) < 1e-12 and rel(a2, b2) < 1e-12
+    a1, a2 = s.d2bydx(f)
+    b1, b2 = o.d2bydx(f)
+    assert rel(a1, b1) < 1e-12 and rel(a2, b2) < 1e-12
+    F = o.forwardfft(f)
+    assert rel(s.forwardfft(f), F) < 1e-13
+    assert rel(s.inversefft(F), o.inversefft(F)) < 1e-13
+    s.close()
