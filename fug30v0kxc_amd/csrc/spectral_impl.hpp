@@ -418,15 +418,21 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
 // host dispatch
 // ---------------------------------------------------------------------------
 // transforms per workgroup: about 256 threads along x; along y, 8 (FP64) or
-// 16 (FP32) column pairs = 128-byte row segments; LDS per workgroup <= 64 KB.
+// 16 (FP32) column pairs = 128-byte row segments.  LDS per workgroup: up to
+// 64 KB, and up to 160 KB (gfx950's limit) for FP64 along y, so that the
+// 512-point FP64 lengths keep full-line column groups (about 78 KB; the 4-field
+// pass at 512^2 x 256 went from 2.15 to 1.48 ms).  In FP32 the 64 KB groups
+// (half lines, paired onto one XCD) measured faster than full-line ones.
+constexpr int SPEC_LDS_X = 65536, SPEC_LDS_Y = 163840;
 template <typename TC, typename PL, int DIR>
 constexpr int spec_b()
 {
     constexpr int tpt = plan_tpt<PL>();
     constexpr int sp = RegFFT<TC, PL>::SP * (int)sizeof(cpx<TC>);
     constexpr int twb = PL::N * (int)sizeof(cpx<TC>);
+    constexpr int lim = (DIR == 1 && sizeof(TC) == 8) ? SPEC_LDS_Y : SPEC_LDS_X;
     int b = DIR == 0 ? (128 / tpt > 0 ? 128 / tpt : 1) : (sizeof(TC) == 8 ? 8 : 16);
-    while (b > 1 && (b * tpt > 1024 || twb + b * sp > 65536)) b /= 2;
+    while (b > 1 && (b * tpt > 1024 || twb + b * sp > lim)) b /= 2;
     return b;
 }
 
@@ -441,9 +447,17 @@ static int launch_one(const SpecLaunch& L, hipStream_t s)
 {
     constexpr int NT = B * plan_tpt<PL>();
     constexpr size_t lds = spec_lds_bytes<TC, PL, B>();
-    static_assert(lds <= 65536, "transform workgroup exceeds 64 KB of LDS");
+    static_assert(lds <= (size_t)SPEC_LDS_Y, "transform workgroup exceeds 160 KB of LDS");
     auto kern = spec_kernel<TC, PL, DIR, B, MODE, LOP>;
     const long items = Lines<PL, DIR, B>::groups(L.g) * L.njobs;
+    // more than 64 KB of dynamic LDS is opted into once per kernel
+    static int lds_ok = lds <= 65536 ? 1 : 0;
+    if (!lds_ok) {
+        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+            return -22;
+        lds_ok = 1;
+    }
     // persistent grid: the workgroups that fit on the device at once
     static int per_cu = 0;
     if (per_cu == 0) {
