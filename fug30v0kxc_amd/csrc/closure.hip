@@ -61,19 +61,31 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ bf16x8 frag(const __bf16* p) { return *(const bf16x8*)p; }
-// tanh-form GELU: g(z) = z (1 + tanh(k (z + a z^3))) / 2, k = sqrt(2/pi), a = 0.044715;
-// tanh(u) = 1 - 2 / (1 + e^{2u}) with the hardware exp and reciprocal
+// tanh-form GELU: g(z) = z (1 + tanh(k (z + a z^3))) / 2, k = sqrt(2/pi), a = 0.044715.
+// With s = 1 / (1 + e^{-2k(z + a z^3)}) (so (1 + tanh) / 2 = s):
+//   g(z) = z s,   g'(z) = s + 2 z s (1 - s) k (1 + 3 a z^2)
+// e^x is taken as the hardware 2^(x log2 e): GE0, GE1 fold -2 log2(e) into k, k a.
 constexpr float GK = 0.7978845608028654f, GKA = 0.035677408136300125f;
-__device__ __forceinline__ float gelu_t(float z)
+constexpr float GE0 = -2.0f * 1.4426950408889634f * 0.7978845608028654f;
+constexpr float GE1 = -2.0f * 1.4426950408889634f * 0.035677408136300125f;
+__device__ __forceinline__ float gelu_s(float z)
 {
-    const float u = z * (GK + GKA * z * z);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u));
+    // z -> -inf: 2^(+inf) = inf, s = 0;  z -> +inf: 2^(-inf) = 0, s = 1
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(z * z, GE1, GE0)));
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + gelu_t(z)); }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
 __device__ __forceinline__ float dgelu(float z)
 {
-    const float t = gelu_t(z);
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * (GK + 3.0f * GKA * z * z);
+    const float s = gelu_s(z);
+    return s + 2.0f * z * s * (1.0f - s) * fmaf(3.0f * GKA, z * z, GK);
+}
+// orders the wave's own LDS accesses (an area no other wave touches): the
+// wave's LDS operations are performed in order, so only the compiler needs fencing
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long x)
@@ -220,6 +232,7 @@ struct FwdArgs {
     int out_f32;
     const float* tn;       // normalised targets [snap][12][n3] for the loss, or null
     double* lossp;         // per-workgroup sum of squared errors
+    int tiled;             // nx and the cell count multiples of 32: neighbourhoods staged in LDS
 };
 
 // the 27 neighbour offsets of a cell, from its (i, j, k) once: x and y wrap, z clamps
@@ -261,45 +274,100 @@ __global__ void __launch_bounds__(FWD_NT, 2) mlp_fwd_kernel(FwdArgs a)
     __shared__ __bf16 sW1[NH * LD1];
     __shared__ __bf16 sW2[NH * LDH];
     __shared__ __bf16 sW3[NOUTP * LDH];
-    __shared__ __bf16 sH[4][CELLS_PER_WAVE * LDH];
+    // per wave: activations [32][LDH], or first the staged neighbourhood of its
+    // 32 cells (3 planes x 3 rows x 34 columns of 16-byte cells)
+    constexpr int TILE = 3 * 3 * (CELLS_PER_WAVE + 2);
+    constexpr int SHW = (TILE * 8 > CELLS_PER_WAVE * LDH) ? TILE * 8 : CELLS_PER_WAVE * LDH;
+    __shared__ __attribute__((aligned(16))) __bf16 sH[4][SHW];
     __shared__ float sLoss[4];
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
-    for (int e = tid; e < NH * NINP / 8; e += FWD_NT) {
-        const int row = e / (NINP / 8), c8 = e - row * (NINP / 8);
+    // weights: every load of the thread issued before its LDS stores
+    static_assert((NH * NINP / 8) % FWD_NT == 0 && (NH * NH / 8) % FWD_NT == 0 && NOUTP * NH / 8 <= FWD_NT, "");
+#pragma unroll
+    for (int it = 0; it < NH * NINP / 8 / FWD_NT; ++it) {
+        const int e = tid + it * FWD_NT, row = e / (NINP / 8), c8 = e - row * (NINP / 8);
         *(bf16x8*)&sW1[row * LD1 + c8 * 8] = ((const bf16x8*)a.w1t)[e];
     }
-    for (int e = tid; e < NH * NH / 8; e += FWD_NT) *(bf16x8*)&sW2[(e >> 3) * LDH + (e & 7) * 8] = ((const bf16x8*)a.w2t)[e];
-    for (int e = tid; e < NOUTP * NH / 8; e += FWD_NT)
-        *(bf16x8*)&sW3[(e >> 3) * LDH + (e & 7) * 8] = ((const bf16x8*)a.w3t)[e];
+#pragma unroll
+    for (int it = 0; it < NH * NH / 8 / FWD_NT; ++it) {
+        const int e = tid + it * FWD_NT;
+        *(bf16x8*)&sW2[(e >> 3) * LDH + (e & 7) * 8] = ((const bf16x8*)a.w2t)[e];
+    }
+    if (tid < NOUTP * NH / 8) *(bf16x8*)&sW3[(tid >> 3) * LDH + (tid & 7) * 8] = ((const bf16x8*)a.w3t)[tid];
     __syncthreads();
 
     __bf16* H = sH[wave];
     const bf16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
     const long ntiles = (a.n + 4 * CELLS_PER_WAVE - 1) / (4 * CELLS_PER_WAVE);
     float se = 0.f;
+    // staged path (a.tiled: nx a multiple of 32, so a wave's 32 cells are one
+    // x-row segment): the 306 cells of their 3x3x3 neighbourhoods, 3 planes x 3
+    // rows x 34 columns, go through the wave's LDS area.  Each lane holds up to
+    // SPL of them for the wave's next tile, loaded while the current one computes.
+    constexpr int SPL = (TILE + 63) / 64;
+    bf16x8 pre[SPL];
+    auto stage_load = [&](long tl) {
+        const long b = tl * (4 * CELLS_PER_WAVE) + wave * CELLS_PER_WAVE;
+        const long c0 = b < a.n ? b : a.n - CELLS_PER_WAVE;   // a wave past the end repeats the last cells
+        const long sb = (c0 / a.n3) * a.n3;
+        const int cc = (int)(c0 - sb), nxy = a.nx * a.ny;
+        const int k = cc / nxy, j = (cc - k * nxy) / a.nx, x0 = cc - k * nxy - j * a.nx;
+#pragma unroll
+        for (int it = 0; it < SPL; ++it) {
+            const int e = l + 64 * it;
+            if (e < TILE) {
+                const int r = e / (CELLS_PER_WAVE + 2), xl = e - r * (CELLS_PER_WAVE + 2);
+                const int dz = r / 3, dy = r - 3 * dz;
+                int kk = k + dz - 1;
+                kk = kk < 0 ? 0 : (kk > a.nz - 1 ? a.nz - 1 : kk);
+                int jj = j + dy - 1;
+                jj = jj < 0 ? jj + a.ny : (jj >= a.ny ? jj - a.ny : jj);
+                int xx = x0 + xl - 1;
+                xx = xx < 0 ? xx + a.nx : (xx >= a.nx ? xx - a.nx : xx);
+                pre[it] = a.grid[sb + (long)kk * nxy + (long)jj * a.nx + xx];
+            }
+        }
+    };
+    if (a.tiled && (long)blockIdx.x < ntiles) stage_load(blockIdx.x);
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long base = tile * (4 * CELLS_PER_WAVE) + wave * CELLS_PER_WAVE;
-        // all 14 neighbour loads of the wave's two row blocks first
         bf16x8 av[2][7];
+        if (a.tiled) {
+            bf16x8* T = (bf16x8*)H;
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-            long c = base + rb * 16 + lr;
-            if (c > a.n - 1) c = a.n - 1;
-            const Nbhd nbh(c, a.n3, a.nx, a.ny, a.nz);
+            for (int it = 0; it < SPL; ++it)
+                if (l + 64 * it < TILE) T[l + 64 * it] = pre[it];
+            if (tile + gridDim.x < ntiles) stage_load(tile + gridDim.x);
+            wave_lds_sync();
 #pragma unroll
-            for (int s = 0; s < 7; ++s) {
-                const int nb = s * 4 + lg;
-                av[rb][s] = nb < 27 ? a.grid[nbh.at(nb)] : zero8;
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int s = 0; s < 7; ++s) {
+                    const int nb = s * 4 + lg;
+                    const int dz = nb / 9, r9 = nb - 9 * dz, dy = r9 / 3, dx = r9 - 3 * dy;
+                    av[rb][s] = nb < 27 ? T[(dz * 3 + dy) * (CELLS_PER_WAVE + 2) + rb * 16 + lr + dx] : zero8;
+                }
+            wave_lds_sync();   // the staging area becomes H
+        } else {
+            // all 14 neighbour loads of the wave's two row blocks first
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                long c = base + rb * 16 + lr;
+                if (c > a.n - 1) c = a.n - 1;
+                const Nbhd nbh(c, a.n3, a.nx, a.ny, a.nz);
+#pragma unroll
+                for (int s = 0; s < 7; ++s) {
+                    const int nb = s * 4 + lg;
+                    av[rb][s] = nb < 27 ? a.grid[nbh.at(nb)] : zero8;
+                }
             }
         }
         // transposed products (W^T X^T): lane l gets units 4(l/16)..+3 of cell l%16,
         // so each lane writes 4 consecutive units of one cell with one 8-byte store
         f32x4 acc[2][4];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) acc[rb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // layer 1
+        for (int mb = 0; mb < 4; ++mb) acc[0][mb] = acc[1][mb] = *(const f32x4*)&a.b1[mb * 16 + lg * 4];
+        // layer 1 (the accumulators start at the bias)
 #pragma unroll
         for (int s = 0; s < 7; ++s)
 #pragma unroll
@@ -310,21 +378,18 @@ __global__ void __launch_bounds__(FWD_NT, 2) mlp_fwd_kernel(FwdArgs a)
             }
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
-            const f32x4 b = *(const f32x4*)&a.b1[mb * 16 + lg * 4];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
                 bf16x4 h;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) h[i] = (__bf16)gelu(acc[rb][mb][i] + b[i]);
+                for (int i = 0; i < 4; ++i) h[i] = (__bf16)gelu(acc[rb][mb][i]);
                 *(bf16x4*)&H[(rb * 16 + lr) * LDH + mb * 16 + lg * 4] = h;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         // layer 2
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) acc[rb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int mb = 0; mb < 4; ++mb) acc[0][mb] = acc[1][mb] = *(const f32x4*)&a.b2[mb * 16 + lg * 4];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const bf16x8 h0 = frag(&H[lr * LDH + s * 32 + lg * 8]);
@@ -336,19 +401,18 @@ __global__ void __launch_bounds__(FWD_NT, 2) mlp_fwd_kernel(FwdArgs a)
                 acc[1][mb] = mfma(w, h1, acc[1][mb]);
             }
         }
-        __syncthreads();
+        wave_lds_sync();
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
-            const f32x4 b = *(const f32x4*)&a.b2[mb * 16 + lg * 4];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
                 bf16x4 h;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) h[i] = (__bf16)gelu(acc[rb][mb][i] + b[i]);
+                for (int i = 0; i < 4; ++i) h[i] = (__bf16)gelu(acc[rb][mb][i]);
                 *(bf16x4*)&H[(rb * 16 + lr) * LDH + mb * 16 + lg * 4] = h;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         // layer 3: lane l gets outputs 4(l/16)..+3 of cell l%16
         f32x4 y[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
@@ -380,7 +444,7 @@ __global__ void __launch_bounds__(FWD_NT, 2) mlp_fwd_kernel(FwdArgs a)
                 }
             }
         }
-        __syncthreads();   // H is rewritten by the next tile
+        wave_lds_sync();   // H is rewritten by the next tile
     }
     if (a.tn) {
 #pragma unroll
@@ -835,8 +899,10 @@ long fwd_grid(long n)
     return tiles < 2L * ncu ? tiles : 2L * ncu;
 }
 
-int launch_fwd(const FwdArgs& a, hipStream_t s)
+int launch_fwd(const FwdArgs& a0, hipStream_t s)
 {
+    FwdArgs a = a0;
+    a.tiled = a.nx % CELLS_PER_WAVE == 0 && a.n % CELLS_PER_WAVE == 0;
     hipLaunchKernelGGL(mlp_fwd_kernel, dim3((unsigned)fwd_grid(a.n)), dim3(FWD_NT), 0, s, a);
     return ok(hipGetLastError()) ? 0 : MHD_E_DEVICE;
 }

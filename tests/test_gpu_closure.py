@@ -55,22 +55,28 @@ def _closure_with_data(nsnap=2, shape=(8, 8, 12), seed=1):
     return cl, snaps
 
 
-def test_forward_matches_oracle():
-    print("[closure] forward", flush=True)
+# (6, 10, 12): neighbourhoods gathered per cell; (3, 5, 64) and (4, 3, 32):
+# rows a multiple of 32 cells long, neighbourhoods staged per x-row segment
+# (x wrap at both row ends, a partly empty last tile in the first)
+@pytest.mark.parametrize("shape", [(6, 10, 12), (3, 5, 64), (4, 3, 32)])
+def test_forward_matches_oracle(shape):
+    print(f"[closure] forward {shape}", flush=True)
     cl, snaps = _closure_with_data()
     p = _params(3)
     cl.params = p
     norm = cl.norm
-    coarse = _fine((6, 10, 12), seed=4)
+    coarse = _fine(shape, seed=4)
     yg = cl.predict(coarse)
     yo = po.mlp_forward(coarse, p.astype(np.float64), norm, round_bf16=True)
     tsd = norm[28:40, None, None, None]
     err = np.abs(yg - yo) / tsd                       # in scaled units
     scale = np.abs((yo - norm[16:28, None, None, None]) / tsd).max()
     # bf16 activations: an FP32-accumulated pre-activation can land on the
-    # other side of a bf16 rounding boundary than the FP64 one, rarely
-    assert np.quantile(err, 0.999) <= 1e-4 * scale
-    assert err.max() <= 2e-2 * scale
+    # other side of a bf16 rounding boundary than the FP64 one, rarely; such a
+    # flip moves all 12 outputs of its cell, so the rare cases are counted per cell
+    cell = err.max(axis=0) / scale
+    assert np.mean(cell > 1e-4) <= 0.01, np.argwhere(cell > 1e-4)[:10]
+    assert cell.max() <= 2e-2
     yn = po.mlp_forward(coarse, p.astype(np.float64), norm, round_bf16=False)
     # the rounding points matter: the unrounded network differs far more
     assert np.abs(yn - yo).max() > 10 * np.quantile(np.abs(yg - yo), 0.5)
