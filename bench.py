@@ -189,9 +189,17 @@ def closure_bench(nfine, seed=20261203):
     fine = Solver(*fine_box, p, precision=MHD_PREC_MIXED)
     fine.static()
     blk = 500
+
+    def note(msg):
+        print(f"[c5 {nfine}] {msg}", file=sys.stderr, flush=True)
+
     # spin-up with the reference's calcdt until t = 0.25
+    nspin = 0
     while fine.scalars()["ttime"] < 0.25:
         fine.step(blk, 0.0, blk)
+        nspin += blk
+        sc = fine.scalars()
+        note(f"spin-up: {nspin} steps, t = {sc['ttime']:.4f}, dt = {sc['dt']:.3e}")
     dtf = fine.scalars()["dt"]
     snaps, nu_f = [], []
     t0 = time.perf_counter()
@@ -199,6 +207,7 @@ def closure_bench(nfine, seed=20261203):
         d = fine.step(blk, dtf, blk)
         snaps.append(closure_targets(fine.get_state(), f, p["f"]))
         nu_f.append(d["nusselt"])
+        note(f"snapshot {len(snaps)} of 6")
     fine.sync()
     fine_ms_per_step = None
     cl = Closure(seed=1)
@@ -207,6 +216,7 @@ def closure_bench(nfine, seed=20261203):
     t1 = time.perf_counter()
     loss = cl.train(6000, batch=4096, lr=1e-3)
     train_s = time.perf_counter() - t1
+    note(f"trained: 6000 steps in {train_s:.2f} s, last batch loss {loss:.4f}")
     r2 = ce.explained_variance(cl.predict(snaps[5][0]), snaps[5][1])
     fwd_ms, train_ms, nfwd = cl.time(batch=8192, reps=20)
     flops_cell = 2 * (216 * 64 + 64 * 64 + 64 * 12)
@@ -216,6 +226,7 @@ def closure_bench(nfine, seed=20261203):
     for _ in range(4):
         d = fine.step(blk, dtf, blk)
         ref.append((closure_targets(fine.get_state(), f, p["f"])[0], d["nusselt"]))
+        note(f"reference block {len(ref)} of 4")
     fine.sync()
     fine_ms_per_step = (time.perf_counter() - t2) / (4 * blk) * 1e3
     fine.close()
@@ -242,6 +253,7 @@ def closure_bench(nfine, seed=20261203):
         ms = (time.perf_counter() - tc) / (nsteps - 2) * 1e3
         runs[mode] = {"spectral_error": [round(e, 4) for e in errs], "mean_spectral_error": round(float(np.mean(errs)), 4),
                       "nusselt": [round(x, 4) for x in nus], "ms_per_step": round(ms, 4)}
+        note(f"coarse {mode}: mean spectral error {runs[mode]['mean_spectral_error']}, {ms:.4f} ms/step")
         co.close()
     cost_fine = fine_ms_per_step * f                 # one coarse step's simulated time
     return {
