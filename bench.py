@@ -272,6 +272,118 @@ def closure_bench(nfine, seed=20261203):
     }
 
 
+def _best(fn, reps=3):
+    """Best wall time (s) of reps calls after one untimed call, and the last result."""
+    out = fn()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        best = min(best, time.perf_counter() - t0)
+    return best, out
+
+
+def aux_bench(cpu=True):
+    """The rows either side of the step: netCDF dumps of a 256x256x160 FP64
+    state (FP64 and FP32 variables, read back), expand to 512x512x160, the
+    factor-4 box filter of a 512x512x384 field, the closure targets of a
+    256x256x192 field set, and the diagnostic and calcdt reductions at
+    256x256x160.  These entry points take host arrays, so their times include
+    the transfers; the kernels alone are in the kernel trace of this command.
+    With cpu, the CPU oracle's expand and box filter are timed on bounded
+    samples beside them."""
+    import tempfile
+    import numpy as np
+    import fug30v0kxc_amd as fa
+    from fug30v0kxc_amd import MHD_PREC_FP64, Solver, closure_targets
+    out = {}
+    rng = np.random.default_rng(7)
+
+    def note(msg):
+        print(f"[aux] {msg}", file=sys.stderr, flush=True)
+
+    # diagnostics and calcdt on a live FP64 box
+    s = Solver(NX, NY, NZ, workload_params(), precision=MHD_PREC_FP64)
+    s.static()
+    s.step(3, DT, 0)
+    s.sync()
+    diag = {}
+    for name, fn in (("GlobalAverages", s.GlobalAverages), ("nusselt", s.nusselt),
+                     ("checkdivb", s.checkdivb), ("calcdt", s.calcdt)):
+        t, _ = _best(lambda: [fn() for _ in range(20)], reps=3)
+        diag[name] = round(t / 20 * 1e6, 1)
+    out["diagnostics_us_per_call"] = diag
+    st = s.get_state()
+    s.close()
+    note("diagnostics done")
+
+    # dumps
+    n3 = NX * NY * NZ
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "T16_001.nc")
+        dumps = {}
+        for iprec in (8, 4):
+            tw, _ = _best(lambda: fa.write_dump(path, st, iteration=3, iprec=iprec))
+            size = os.path.getsize(path)
+            tr, (back, _) = _best(lambda: fa.read_dump(path))
+            if iprec == 8:
+                exact = all(np.array_equal(back[k], st[k]) for k in fa.DUMP_FIELDS)
+            else:
+                exact = all(np.array_equal(back[k], st[k].astype(np.float32).astype(np.float64))
+                            for k in fa.DUMP_FIELDS)
+            dumps[f"fp{8 * iprec}"] = {"bytes": size, "write_ms": round(tw * 1e3, 1),
+                                       "write_gbs": round(size / tw / 1e9, 2), "read_ms": round(tr * 1e3, 1),
+                                       "read_gbs": round(size / tr / 1e9, 2), "round_trip_exact": bool(exact)}
+        out["dump_256x256x160"] = dumps
+    note("dumps done")
+
+    # expand 256x256x160 -> 512x512x160 (one field)
+    f = st["t"]
+    te, e = _best(lambda: fa.expand(f))
+    # without Nyquist content (which expand keeps at full amplitude as a fine-grid
+    # cosine, interpolate/fft.F90:888, 913) the coarse points are fine points:
+    # check that on a band-limited field of the same shape
+    spec = np.fft.rfft2(rng.standard_normal(f.shape))
+    spec[:, NY // 2, :] = 0.0
+    spec[:, :, NX // 2] = 0.0
+    bl = np.fft.irfft2(spec, s=(NY, NX))
+    rt = float(np.abs(fa.expand(bl)[:, ::2, ::2] - bl).max() / np.abs(bl).max())
+    out["expand_256x256x160"] = {"ms": round(te * 1e3, 2), "out_cells": int(e.size),
+                                 "host_bytes": int(8 * (f.size + e.size)),
+                                 "host_inclusive_gbs": round(8 * (f.size + e.size) / te / 1e9, 2),
+                                 "coarse_points_rel_err": float(f"{rt:.2e}")}
+    del e
+    note("expand done")
+
+    # box filter 512x512x384 -> 128x128x96
+    g = rng.standard_normal((384, 512, 512))
+    tb, b = _best(lambda: fa.box_filter(g, 4))
+    out["box_filter_512x512x384_f4"] = {"ms": round(tb * 1e3, 2), "in_cells": int(g.size),
+                                        "mean_rel_err": float(f"{abs(b.mean() - g.mean()) / g.std():.2e}")}
+    note("box filter done")
+
+    # closure targets of 8 fields at 256x256x192, factor 4
+    fine = {k: rng.standard_normal((192, 256, 256)) for k in fa.CLOSURE_FIELDS}
+    tt, _ = _best(lambda: closure_targets(fine, 4, 0.5))
+    out["closure_targets_256x256x192_f4"] = {"ms": round(tt * 1e3, 2), "in_cells": 192 * 256 * 256}
+    note("targets done")
+
+    if cpu:
+        from oracle import pyoracle
+        nth = pyoracle.threads()
+        fs = np.ascontiguousarray(f[:16])
+        tc, _ = _best(lambda: pyoracle.expand(fs), reps=1)
+        tcb, _ = _best(lambda: pyoracle.box_filter(g, 4), reps=1)
+        out["cpu_baseline"] = {"kind": "port", "cores": nth,
+                               "expand_ms_per_plane": round(tc / 16 * 1e3, 3),
+                               "expand_ms_full_field_equiv": round(tc / 16 * NZ * 1e3, 1),
+                               "box_filter_512x512x384_ms": round(tcb * 1e3, 1),
+                               "sample": f"CPU oracle expand of 16 of the {NZ} planes (scaled to the field) "
+                                         f"and box filter of the whole 512x512x384 field, {nth} threads"}
+        note("cpu baseline done")
+    return out
+
+
 def advance(r, total, dt):
     """Step a run on to `total` steps; returns the last diagnostics."""
     left = total - r["done"]
@@ -323,9 +435,10 @@ def main():
                     help="kernel family for the roofline object")
     ap.add_argument("--c5-fine", type=int, default=256,
                     help="c5 (closure) workload: fine nx = ny (nz = 3/4 of it); 512 is the 512^2 x 384 case")
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "aux"],
                     help="c2: 256x256x160 (default); c3: 512x512x256 with the eta(z) step; "
-                         "c4: ensemble of 32 boxes in hipGraph replays")
+                         "c4: ensemble of 32 boxes in hipGraph replays; c5: learned closure; "
+                         "aux: dumps, expand, box filter, closure targets, diagnostics")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--err-steps", type=int, default=ERR_STEPS,
                     help="steps after which the mixed run is compared with the FP64 run")
@@ -360,6 +473,19 @@ def main():
                "config": {"workload": "ensemble of 32 seeded boxes, mixed precision, hipGraph replays",
                           **{k: v for k, v in r.items() if k not in ("graph_mcell_steps_per_s",)}},
                "roofline": None, "cpu_baseline": None}
+        print(json.dumps(out))
+        return 0
+    if args.workload == "aux":
+        r = aux_bench(cpu=not args.no_cpu_baseline)
+        ex = r["expand_256x256x160"]
+        out = {"metric": "expand 256x256x160 -> 512x512x160 output cells per second, host arrays in and out",
+               "value": round(ex["out_cells"] / (ex["ms"] * 1e-3) / 1e6, 1), "unit": "Mcell/s", "n_gpus": 1,
+               "steps": None, "warmup": None, "ms_per_step": None, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": {"workload": "rows either side of the step (dumps, expand, box filter, "
+                                      "closure targets, diagnostics)", **{k: v for k, v in r.items()
+                                                                          if k != "cpu_baseline"}},
+               "roofline": None, "cpu_baseline": r.get("cpu_baseline")}
         print(json.dumps(out))
         return 0
     if args.workload == "c5":

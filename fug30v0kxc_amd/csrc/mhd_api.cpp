@@ -1031,7 +1031,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->eta_on = p->eta_c > 1.0;
     bytes += 2 * vz + (c->eta_on ? 3 * fw : 0);       // eta(z) profile, J
     bytes += W_COUNT * fw;                            // work
-    bytes += al256(nz * 8 * sizeof(double)) + 2 * al256(8 * sizeof(double));  // reductions
+    bytes += al256(nz * (RED_CHUNKS + 1) * 8 * sizeof(double)) + 2 * al256(8 * sizeof(double));  // reductions
     if (hipMalloc((void**)&c->pool, bytes) != hipSuccess) {
         (void)hipStreamDestroy(c->s);
         delete c;
@@ -1071,7 +1071,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     for (int f = 0; f < 2; ++f)
         for (int l = 0; l < 3; ++l) c->mhist[f][l] = (double*)take(vz);
     for (int i = 0; i < W_COUNT; ++i) c->wk[i] = take(fw);
-    c->red_partial = (double*)take(al256(nz * 8 * sizeof(double)));
+    c->red_partial = (double*)take(al256(nz * (RED_CHUNKS + 1) * 8 * sizeof(double)));
     c->red_result = (double*)take(al256(8 * sizeof(double)));
     c->d_bzbar = (double*)take(al256(8 * sizeof(double)));
     c->t0 = (double*)take(vz);

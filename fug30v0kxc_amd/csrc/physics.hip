@@ -774,14 +774,17 @@ template <typename TW>
 __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
 {
     __shared__ double sh[5 * NT];
-    const int k = blockIdx.x;
+    // workgroup = (plane k, chunk ch): the points [lo, hi) of the plane
+    const int k = blockIdx.x / RED_CHUNKS, ch = blockIdx.x - k * RED_CHUNKS;
     const long base = (long)k * a.g.nxy;
     const int m = a.g.nxy;
+    const int lo = (int)((long)ch * m / RED_CHUNKS), hi = (int)((long)(ch + 1) * m / RED_CHUNKS);
     const int tid = threadIdx.x;
+    double* part = a.partial + (long)blockIdx.x * 8;
     if (a.op == RED_GLOBALAVG) {
         // src/utils.F90:55-82: rav, urav, vrav, keav, bsqav
         double v[5] = {0, 0, 0, 0, 0};
-        for (int i = tid; i < m; i += NT) {
+        for (int i = lo + tid; i < hi; i += NT) {
             const long q = base + i;
             const double uu = a.u[q], vv = a.v[q], rr = a.r[q], ww = a.w[q];
             double bx = rdw<TW>(a.bx, q), by = rdw<TW>(a.by, q), bz = rdw<TW>(a.bz, q);
@@ -799,10 +802,10 @@ __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
         const int kind[5] = {0, 0, 0, 0, 0};
         block_reduce<5>(v, kind, sh);
         if (tid == 0)
-            for (int i = 0; i < 5; ++i) a.partial[k * 8 + i] = v[i];
+            for (int i = 0; i < 5; ++i) part[i] = v[i];
     } else if (a.op == RED_MINMAX || a.op == RED_DIVB) {
         double v[2] = {1e300, -1e300};
-        for (int i = tid; i < m; i += NT) {
+        for (int i = lo + tid; i < hi; i += NT) {
             const long q = base + i;
             double x;
             if (a.op == RED_MINMAX)
@@ -818,14 +821,14 @@ __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
         const int kind[2] = {1, 2};
         block_reduce<2>(v, kind, sh);
         if (tid == 0) {
-            a.partial[k * 8 + 0] = v[0];
-            a.partial[k * 8 + 1] = v[1];
+            part[0] = v[0];
+            part[1] = v[1];
         }
     } else if (a.op == RED_CALCDT) {
         // src/utils.F90:168-186
         double v[3] = {1e300, -1.0e-50, -1.0e-50};
         const double f = a.p.f, g1 = a.p.g1;
-        for (int i = tid; i < m; i += NT) {
+        for (int i = lo + tid; i < hi; i += NT) {
             const long q = base + i;
             const double u0 = a.u[q], v0 = a.v[q], w0 = a.w[q], r0 = a.r[q], t0 = a.t[q];
             double bz0 = rdw<TW>(a.bz, q), bx0 = rdw<TW>(a.bx, q), by0 = rdw<TW>(a.by, q);
@@ -844,26 +847,46 @@ __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
         const int kind[3] = {1, 2, 2};
         block_reduce<3>(v, kind, sh);
         if (tid == 0)
-            for (int i = 0; i < 3; ++i) a.partial[k * 8 + i] = v[i];
+            for (int i = 0; i < 3; ++i) part[i] = v[i];
     } else if (a.op == RED_NUSSELT) {
-        // src/dbydz.F90:4878-4886, bottom plane only (grid of 1)
+        // src/dbydz.F90:4878-4886, bottom plane only (RED_CHUNKS workgroups)
         double v[1] = {0.0};
         const long L = (long)(a.g.nz - 1) * m;
         const double c = 1.0 / (6.0 * a.p.dz);
-        for (int i = tid; i < m; i += NT)
+        for (int i = lo + tid; i < hi; i += NT)
             v[0] += c * (11.0 * a.t[L + i] - 18.0 * a.t[L - m + i] + 9.0 * a.t[L - 2 * m + i] - 2.0 * a.t[L - 3 * m + i]);
         const int kind[1] = {0};
         block_reduce<1>(v, kind, sh);
-        if (tid == 0) a.partial[0] = v[0];
+        if (tid == 0) part[0] = v[0];
     }
 }
 
-__global__ void reduce_finish_kernel(RedArgs a)
+__global__ void __launch_bounds__(NT) reduce_finish_kernel(RedArgs a)
 {
-    if (threadIdx.x != 0) return;
     const int nz = a.g.nz;
-    const double* P = a.partial;
     double* R = a.result;
+    // per plane: its chunks combined in order (sums for the averages and
+    // Nusselt, min / max for the others), one (plane, value) per thread, into
+    // P[k * 8 + i]: in LDS when the planes fit, else past the chunk partials
+    constexpr int LDS_VALS = 4096;
+    __shared__ double sP[LDS_VALS];
+    const int nk = a.op == RED_NUSSELT ? 1 : nz;
+    double* P = nk * 8 <= LDS_VALS ? sP : a.partial + (long)nz * RED_CHUNKS * 8;
+    const int nv = a.op == RED_GLOBALAVG ? 5 : (a.op == RED_CALCDT ? 3 : (a.op == RED_NUSSELT ? 1 : 2));
+    const bool sum = a.op == RED_GLOBALAVG || a.op == RED_NUSSELT;
+    for (int idx = threadIdx.x; idx < nk * nv; idx += blockDim.x) {
+        const int k = idx / nv, i = idx - k * nv;
+        const double* c = a.partial + (long)k * RED_CHUNKS * 8 + i;
+        double y[RED_CHUNKS];
+#pragma unroll
+        for (int ch = 0; ch < RED_CHUNKS; ++ch) y[ch] = c[ch * 8];
+        double x = y[0];
+#pragma unroll
+        for (int ch = 1; ch < RED_CHUNKS; ++ch) x = sum ? x + y[ch] : (i == 0 ? fmin(x, y[ch]) : fmax(x, y[ch]));
+        P[k * 8 + i] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     if (a.op == RED_GLOBALAVG) {
         // src/utils.F90:83-127
         double urbar = 0, vrbar = 0, rbar = 0, me = 0, ke = 0;
@@ -929,10 +952,10 @@ __global__ void reduce_finish_kernel(RedArgs a)
 
 int launch_reduce(const RedArgs& a, hipStream_t s)
 {
-    const unsigned g = (a.op == RED_NUSSELT) ? 1u : (unsigned)a.g.nz;
+    const unsigned g = (a.op == RED_NUSSELT) ? (unsigned)RED_CHUNKS : (unsigned)(a.g.nz * RED_CHUNKS);
     if (a.wf32) hipLaunchKernelGGL(reduce_plane_kernel<float>, dim3(g), dim3(NT), 0, s, a);
     else hipLaunchKernelGGL(reduce_plane_kernel<double>, dim3(g), dim3(NT), 0, s, a);
-    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(NT), 0, s, a);
     return hipGetLastError() == hipSuccess ? 0 : -22;
 }
 

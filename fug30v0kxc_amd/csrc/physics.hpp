@@ -165,9 +165,13 @@ struct RedArgs {
     int wf32;                   // bx/by/bz are float
     const double *bxm, *bym;    // mixed: means of bx, by (null: B holds them)
     double bzm;                 // mixed: mean of bz
-    double* partial;            // nz * 8
+    double* partial;            // nz * (RED_CHUNKS + 1) * 8
     double* result;             // 8 doubles
 };
+// workgroups per plane of a reduction: nz * RED_CHUNKS workgroups fill the
+// device (nz alone, 160 at 256x256x160, leaves CUs idle); the finishing
+// kernel combines the chunks of a plane in order, then the planes
+constexpr int RED_CHUNKS = 8;
 int launch_reduce(const RedArgs& a, hipStream_t s);
 // subtract result[0], result[1] from u, v (GlobalAverages side effect)
 int launch_com_shift(Geom g, double* u, double* v, const double* res, hipStream_t s);
