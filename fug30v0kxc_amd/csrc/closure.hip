@@ -705,12 +705,15 @@ __global__ void __launch_bounds__(256) grad_reduce_kernel(const float* partial, 
                                                           float* grad, double* loss)
 {
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    // the sums run in wave order; unrolled, so a thread keeps 16 loads in flight
     if (p < NPARAM) {
         double s = 0.0;
+#pragma unroll 16
         for (long w = 0; w < nw; ++w) s += (double)partial[w * NPARAM + p];
         grad[p] = (float)s;
     } else if (p == NPARAM) {
         double s = 0.0;
+#pragma unroll 16
         for (long w = 0; w < nw; ++w) s += (double)lossp[w];
         *loss = s;
     }

@@ -10,10 +10,17 @@
 // over (nz, ny, nx) as double (iprec 8) or float (iprec 4).  The reader
 // accepts any classic file holding those variables (double or float) and
 // any attribute/dimension order.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mhd.h"
@@ -96,6 +103,20 @@ struct In {
         o += bytes;
     }
 };
+
+// fn(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads
+// (the byte-order conversions of the field arrays)
+template <typename F>
+void par_for(size_t n, F fn)
+{
+    unsigned t = std::thread::hardware_concurrency();
+    t = t < 1 ? 1 : (t > 16 ? 16 : t);
+    if (n < ((size_t)1 << 20)) t = 1;
+    std::vector<std::thread> th;
+    for (unsigned i = 1; i < t; ++i) th.emplace_back(fn, n * i / t, n * (i + 1) / t);
+    fn((size_t)0, n / t);
+    for (auto& x : th) x.join();
+}
 
 double be_f64(const unsigned char* q)
 {
@@ -197,6 +218,8 @@ int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const
         for (int s = 0; s < 4; ++s) h.b[begin_pos[i] + s] = (unsigned char)(b >> (24 - 8 * s));
         off += vars[i].vsize;
     }
+    // fields converted to big-endian on the host threads, one buffer per
+    // field (a mapped output file measured slower on the GPU hosts)
     FILE* fp = fopen(path, "wb");
     if (!fp) return -13;
     bool ok = fwrite(h.b.data(), 1, h.b.size(), fp) == h.b.size();
@@ -205,51 +228,71 @@ int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const
     d.i32(meta->iteration);
     d.f64(meta->timestep);
     ok = ok && fwrite(d.b.data(), 1, d.b.size(), fp) == d.b.size();
-    std::vector<unsigned char> buf(fsize + 4);
+    const size_t padded = (fsize + 3) & ~(size_t)3;
+    std::vector<unsigned char> buf(padded);
     for (int f = 0; f < 8 && ok; ++f) {
         const double* src = fields[f];
+        unsigned char* bp = buf.data();
         if (iprec == 8) {
-            for (size_t i = 0; i < n3; ++i) {
-                uint64_t v;
-                memcpy(&v, &src[i], 8);
-                for (int s = 0; s < 8; ++s) buf[8 * i + s] = (unsigned char)(v >> (56 - 8 * s));
-            }
+            par_for(n3, [=](size_t lo, size_t hi) {
+                for (size_t i = lo; i < hi; ++i) {
+                    uint64_t v;
+                    memcpy(&v, &src[i], 8);
+                    v = __builtin_bswap64(v);
+                    memcpy(bp + 8 * i, &v, 8);
+                }
+            });
         } else {
-            for (size_t i = 0; i < n3; ++i) {
-                const float x = (float)src[i];
-                uint32_t v;
-                memcpy(&v, &x, 4);
-                for (int s = 0; s < 4; ++s) buf[4 * i + s] = (unsigned char)(v >> (24 - 8 * s));
-            }
+            par_for(n3, [=](size_t lo, size_t hi) {
+                for (size_t i = lo; i < hi; ++i) {
+                    const float x = (float)src[i];
+                    uint32_t v;
+                    memcpy(&v, &x, 4);
+                    v = __builtin_bswap32(v);
+                    memcpy(bp + 4 * i, &v, 4);
+                }
+            });
         }
-        const size_t padded = (fsize + 3) & ~(size_t)3;
-        for (size_t i = fsize; i < padded; ++i) buf[i] = 0;
-        ok = fwrite(buf.data(), 1, padded, fp) == padded;
+        for (size_t i = fsize; i < padded; ++i) bp[i] = 0;
+        ok = fwrite(bp, 1, padded, fp) == padded;
     }
     ok = (fclose(fp) == 0) && ok;
     return ok ? MHD_OK : -14;
 }
 
-static int read_file(const char* path, std::vector<unsigned char>& data)
+// the file mapped read-only: a metadata query touches only the header pages
+struct Mapped {
+    const unsigned char* p = nullptr;
+    size_t n = 0;
+    ~Mapped()
+    {
+        if (p) munmap((void*)p, n);
+    }
+    const unsigned char* data() const { return p; }
+    size_t size() const { return n; }
+};
+
+static int map_file(const char* path, Mapped& m)
 {
-    FILE* fp = fopen(path, "rb");
-    if (!fp) return -13;
-    fseek(fp, 0, SEEK_END);
-    long sz = ftell(fp);
-    fseek(fp, 0, SEEK_SET);
-    if (sz < 8) { fclose(fp); return -15; }
-    data.resize((size_t)sz);
-    size_t got = fread(data.data(), 1, (size_t)sz, fp);
-    fclose(fp);
-    return got == (size_t)sz ? 0 : -15;
+    const int fd = open(path, O_RDONLY);
+    if (fd < 0) return -13;
+    struct stat st;
+    if (fstat(fd, &st) != 0) { close(fd); return -13; }
+    if (st.st_size < 8) { close(fd); return -15; }
+    void* p = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+    close(fd);
+    if (p == MAP_FAILED) return -13;
+    m.p = (const unsigned char*)p;
+    m.n = (size_t)st.st_size;
+    return 0;
 }
 
 // fields may be NULL to query meta only (dimensions, attributes, scalars)
 int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8])
 {
     if (!path || !meta) return MHD_E_ARG;
-    std::vector<unsigned char> data;
-    int e = read_file(path, data);
+    Mapped data;
+    int e = map_file(path, data);
     if (e) return e;
     In in{data.data(), data.size()};
     if (memcmp(data.data(), "CDF\x01", 4) != 0) return -15;
@@ -278,10 +321,12 @@ int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8])
             int type = (int)in.u32();
             uint32_t count = in.u32();
             const unsigned char* q = data.data() + in.o;
+            if (in.o + 8 > data.size()) return -15;   // the value reads below stay inside the file
             if (nm == "version" && type == NC_INT && count == 1)
                 meta->version = (int)(((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3]);
             if (nm == "history" && type == NC_CHAR) {
                 size_t L = count < sizeof(meta->history) - 1 ? count : sizeof(meta->history) - 1;
+                if (in.o + L > data.size()) return -15;
                 memcpy(meta->history, q, L);
                 meta->history[L] = 0;
             }
@@ -309,7 +354,7 @@ int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8])
         int type = (int)in.u32();
         in.u32();  // vsize
         size_t begin = in.u32();
-        if (!in.ok || begin >= data.size()) return -15;
+        if (!in.ok || begin + 8 > data.size()) return -15;
         const unsigned char* q = data.data() + begin;
         if (nm == "time" && type == NC_DOUBLE) meta->time = be_f64(q);
         if (nm == "timestep" && type == NC_DOUBLE) meta->timestep = be_f64(q);
@@ -322,9 +367,25 @@ int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8])
             if (fields && fields[f]) {
                 double* dst = fields[f];
                 if (type == NC_DOUBLE)
-                    for (size_t k = 0; k < n3; ++k) dst[k] = be_f64(q + 8 * k);
+                    par_for(n3, [=](size_t lo, size_t hi) {
+                        for (size_t k = lo; k < hi; ++k) {
+                            uint64_t v;
+                            memcpy(&v, q + 8 * k, 8);
+                            v = __builtin_bswap64(v);
+                            memcpy(&dst[k], &v, 8);
+                        }
+                    });
                 else
-                    for (size_t k = 0; k < n3; ++k) dst[k] = (double)be_f32(q + 4 * k);
+                    par_for(n3, [=](size_t lo, size_t hi) {
+                        for (size_t k = lo; k < hi; ++k) {
+                            uint32_t v;
+                            memcpy(&v, q + 4 * k, 4);
+                            v = __builtin_bswap32(v);
+                            float x;
+                            memcpy(&x, &v, 4);
+                            dst[k] = (double)x;
+                        }
+                    });
             }
         }
     }
