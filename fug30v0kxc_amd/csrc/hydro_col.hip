@@ -217,16 +217,18 @@ __global__ void __launch_bounds__(HNT) hydro_col_kernel(HydroColArgs av)
             ((TW*)a.h3z[0])[q] = (TW)0;
             ((TW*)a.h3z[1])[q] = (TW)0;
         }
-        ((TW*)a.udot)[q] = fu;
-        ((TW*)a.vdot)[q] = fv;
-        ((TW*)a.wdot)[q] = fw;
-        ((TW*)a.rdot)[q] = rd;
-        ((TW*)a.tdot)[q] = ft;
-        a.xnew[0][q] = xu;
-        a.xnew[1][q] = xv;
-        a.xnew[2][q] = xw;
-        a.xnew[3][q] = xr;
-        a.xnew[4][q] = xt;
+        // the RHS levels and the new state are next read a step or more later,
+        // after the step has streamed other fields: stored non-temporally
+        __builtin_nontemporal_store(fu, (TW*)a.udot + q);
+        __builtin_nontemporal_store(fv, (TW*)a.vdot + q);
+        __builtin_nontemporal_store(fw, (TW*)a.wdot + q);
+        __builtin_nontemporal_store(rd, (TW*)a.rdot + q);
+        __builtin_nontemporal_store(ft, (TW*)a.tdot + q);
+        __builtin_nontemporal_store(xu, a.xnew[0] + q);
+        __builtin_nontemporal_store(xv, a.xnew[1] + q);
+        __builtin_nontemporal_store(xw, a.xnew[2] + q);
+        __builtin_nontemporal_store(xr, a.xnew[3] + q);
+        __builtin_nontemporal_store(xt, a.xnew[4] + q);
         if constexpr (MIX) {
             a.snew[0][q] = (float)xu;
             a.snew[1][q] = (float)xv;

@@ -45,6 +45,19 @@ __device__ __forceinline__ void st_pair(TC* __restrict__ p, long i, TC a, TC b)
     v.y = b;
     *reinterpret_cast<vec2_t<TC>*>(p + i) = v;
 }
+// derivative outputs are read back only by later kernels, after the step has
+// streamed other fields through the caches: stored non-temporally
+template <typename TC>
+__device__ __forceinline__ void st_pair_nt(TC* __restrict__ p, long i, TC a, TC b)
+{
+    __builtin_nontemporal_store(a, p + i);
+    __builtin_nontemporal_store(b, p + i + 1);
+}
+template <typename TC>
+__device__ __forceinline__ void st_nt(TC* __restrict__ p, long i, TC a)
+{
+    __builtin_nontemporal_store(a, p + i);
+}
 
 // Input loaders, fixed at compile time per launch.  Inputs have the compute
 // type TC, except LD_BG, which reads the FP64 state and subtracts a per-plane
@@ -400,10 +413,10 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
 #pragma unroll
                         for (int r = 0; r < R0; ++r) {
                             const int n = j + r * NB0;
-                            if constexpr (DIR == 1) st_pair(out, ia + n * st, v[r].x + add, v[r].y + add);
+                            if constexpr (DIR == 1) st_pair_nt(out, ia + n * st, v[r].x + add, v[r].y + add);
                             else {
-                                out[ia + n * st] = v[r].x + add;
-                                out[ib + n * st] = v[r].y + add;
+                                st_nt(out, ia + n * st, v[r].x + add);
+                                st_nt(out, ib + n * st, v[r].y + add);
                             }
                         }
                     }
