@@ -230,11 +230,11 @@ __global__ void __launch_bounds__(HNT) hydro_col_kernel(HydroColArgs av)
         __builtin_nontemporal_store(xr, a.xnew[3] + q);
         __builtin_nontemporal_store(xt, a.xnew[4] + q);
         if constexpr (MIX) {
-            a.snew[0][q] = (float)xu;
-            a.snew[1][q] = (float)xv;
-            a.snew[2][q] = (float)xw;
-            a.snew[3][q] = (float)(xr - a.r0[k]);
-            a.snew[4][q] = (float)(xt - a.t0[k]);
+            __builtin_nontemporal_store((float)xu, a.snew[0] + q);
+            __builtin_nontemporal_store((float)xv, a.snew[1] + q);
+            __builtin_nontemporal_store((float)xw, a.snew[2] + q);
+            __builtin_nontemporal_store((float)(xr - a.r0[k]), a.snew[3] + q);
+            __builtin_nontemporal_store((float)(xt - a.t0[k]), a.snew[4] + q);
         }
     };
 
