@@ -113,7 +113,13 @@ void par_for(size_t n, F fn)
     t = t < 1 ? 1 : (t > 16 ? 16 : t);
     if (n < ((size_t)1 << 20)) t = 1;
     std::vector<std::thread> th;
-    for (unsigned i = 1; i < t; ++i) th.emplace_back(fn, n * i / t, n * (i + 1) / t);
+    for (unsigned i = 1; i < t; ++i) {
+        try {
+            th.emplace_back(fn, n * i / t, n * (i + 1) / t);
+        } catch (...) {
+            fn(n * i / t, n * (i + 1) / t);   // no thread to be had: this chunk here
+        }
+    }
     fn((size_t)0, n / t);
     for (auto& x : th) x.join();
 }

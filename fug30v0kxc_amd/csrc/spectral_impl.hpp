@@ -50,8 +50,11 @@ __device__ __forceinline__ void st_pair(TC* __restrict__ p, long i, TC a, TC b)
 template <typename TC>
 __device__ __forceinline__ void st_pair_nt(TC* __restrict__ p, long i, TC a, TC b)
 {
-    __builtin_nontemporal_store(a, p + i);
-    __builtin_nontemporal_store(b, p + i + 1);
+    using v2 = TC __attribute__((ext_vector_type(2)));
+    v2 v;
+    v.x = a;
+    v.y = b;
+    __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p + i));
 }
 template <typename TC>
 __device__ __forceinline__ void st_nt(TC* __restrict__ p, long i, TC a)
