@@ -16,7 +16,10 @@ energy) are reported under "mixed".
 
 Prints one JSON line.  The roofline object covers the dominant kernel family
 (the x-direction spectral first+second derivative of T, u, v, w), timed with
-HIP events on the solver's stream over the timed steps.  cpu_baseline times
+HIP events on the solver's stream over the timed steps.  step_roofline puts
+the whole step against the same peak at its algorithmic bytes per cell-step
+(139 working-precision array passes, the FP64 state read and written, the RHS
+history: 1448 B in FP64, 780 B mixed).  cpu_baseline times
 the CPU oracle (a C restatement of the reference) on a bounded sample of the
 same workload: a few steps of the full box on all host threads, and one
 step on one thread.
@@ -407,6 +410,21 @@ def measured_traffic(precision, kernel):
     return None
 
 
+# algorithmic bytes per cell-step of the whole step: 139 working-precision array
+# passes, the FP64 state (7 in, 7 out) and the RHS history (21 in, 7 out)
+def step_bytes_per_cell(elem):
+    return 139 * elem + 14 * 8 + 28 * elem
+
+
+def step_roofline(cells, ms_per_step, elem):
+    """The whole step against the HBM peak, at the algorithmic bytes per cell-step."""
+    b = step_bytes_per_cell(elem)
+    achieved = b * cells / (ms_per_step * 1e-3) / 1e9
+    return {"bound": "hbm", "bytes_per_cell_step": b, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "ceiling_mcell_steps_per_s": round(HBM_PEAK_GBS * 1e9 / b / 1e6, 1)}
+
+
 def roofline_of(r, kernel, elem, workload="c2"):
     cells = r["cells"]
     # algorithmic bytes of one launch: 4 fields, 1 read + 2 writes each
@@ -516,6 +534,7 @@ def main():
                "config": {"workload": desc + ", mixed precision, diagnostics every 100 steps",
                           "nx": nx, "ny": ny, "nz": nz, "dt": dt, "ndu06": NDU06, "precision": "mixed"},
                "roofline": roofline_of(rm, args.kernel, 4, args.workload), "cpu_baseline": None,
+               "step_roofline": step_roofline(cells, rm["elapsed"] / args.steps * 1e3, 4),
                "device_bytes": rm["device_bytes"]}
         rm["solver"].close()
         print(json.dumps(out))
@@ -547,6 +566,7 @@ def main():
                                         for k, v in qe.items()},
                                      "fields": {k: float(f"{v:.3e}") for k, v in fe.items()}},
                  "roofline": roofline_of(rm, args.kernel, 4, args.workload),
+                 "step_roofline": step_roofline(cells, rm["elapsed"] / args.steps * 1e3, 4),
                  "tune": rm.get("tune"),
                  "device_bytes": rm["device_bytes"]}
         rm["solver"].close()
@@ -566,6 +586,7 @@ def main():
                       "nx": nx, "ny": ny, "nz": nz, "dt": dt, "ndu06": NDU06,
                       "precision": "fp64"},
            "roofline": roofline_of(r64, args.kernel, 8, args.workload), "cpu_baseline": cpu,
+           "step_roofline": step_roofline(cells, r64["elapsed"] / args.steps * 1e3, 8),
            "mixed": mixed,
            "tune": r64.get("tune"),
            "device_bytes": r64["device_bytes"],
