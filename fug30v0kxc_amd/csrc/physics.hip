@@ -401,61 +401,87 @@ __global__ void __launch_bounds__(NT) bandj_spec_kernel(MagArgs a)
 
 // J^2, src/magnetic.f90:630-645 (dybzh/dxbzh/jzh hold physical-space fields here)
 // J^2 (src/magnetic.f90:596-645) and, for the eta(z) extension, J itself.
-// Column walk with 5-plane windows of bx, by for their d/dz (flag 5).
+// Column walk with 5-plane windows of bx, by for their d/dz (flag 5).  A
+// thread owns two neighbouring columns (nx is even), so every load and store
+// moves two elements.
 template <typename TW>
 __global__ void __launch_bounds__(NT) jsq_kernel(MagArgs av)
 {
     (void)av;
     const MagArgs& a = kernarg<MagArgs>();
+    using TW2 = TW __attribute__((ext_vector_type(2)));
     const int nxy = a.g.nxy, nz = a.g.nz;
-    const long col = (long)blockIdx.x * NT + threadIdx.x;
+    const long col = 2 * ((long)blockIdx.x * NT + threadIdx.x);
     if (col >= nxy) return;
     const int k0 = blockIdx.y * a.zc;
     const int k1 = (k0 + a.zc < nz) ? k0 + a.zc : nz;
     const TW* by = (const TW*)a.by;
     const TW* bx = (const TW*)a.bx;
     const double h = a.p.ho12;
-    double yw[5], xw[5];
-    bool primed = false;
+    auto ld2 = [](const void* p, long q) { return *(const TW2*)((const TW*)p + q); };
+    // interior planes [2, nz-2) use the centred stencil from the windows
+    const int ki0 = k0 > 2 ? k0 : 2, ki1 = k1 < nz - 2 ? k1 : nz - 2;
+    double yw[2][5], xw[2][5];
+    if (ki0 < ki1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long q = col + (long)(ki0 - 2 + j) * nxy;
+            const TW2 y2 = ld2(by, q), x2 = ld2(bx, q);
+            yw[0][j + 1] = (double)y2.x; yw[1][j + 1] = (double)y2.y;
+            xw[0][j + 1] = (double)x2.x; xw[1][j + 1] = (double)x2.y;
+        }
+    }
     for (int k = k0; k < k1; ++k) {
         const long q = col + (long)k * nxy;
-        double wk1, wk2;
-        if (k >= 2 && k <= nz - 3) {
-            if (!primed) {
+        double wk1[2], wk2[2];
+        if (k >= ki0 && k < ki1) {
+            const TW2 y2 = ld2(by, q + 2L * nxy), x2 = ld2(bx, q + 2L * nxy);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    yw[j + 1] = (double)by[q + (long)(j - 2) * nxy];
-                    xw[j + 1] = (double)bx[q + (long)(j - 2) * nxy];
+                    yw[c][j] = yw[c][j + 1];
+                    xw[c][j] = xw[c][j + 1];
                 }
-                primed = true;
             }
+            yw[0][4] = (double)y2.x; yw[1][4] = (double)y2.y;
+            xw[0][4] = (double)x2.x; xw[1][4] = (double)x2.y;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                yw[j] = yw[j + 1];
-                xw[j] = xw[j + 1];
+            for (int c = 0; c < 2; ++c) {
+                wk1[c] = h * (yw[c][0] - 8.0 * yw[c][1] + 8.0 * yw[c][3] - yw[c][4]);
+                wk2[c] = h * (xw[c][0] - 8.0 * xw[c][1] + 8.0 * xw[c][3] - xw[c][4]);
             }
-            yw[4] = (double)by[q + 2L * nxy];
-            xw[4] = (double)bx[q + 2L * nxy];
-            wk1 = h * (yw[0] - 8.0 * yw[1] + 8.0 * yw[3] - yw[4]);
-            wk2 = h * (xw[0] - 8.0 * xw[1] + 8.0 * xw[3] - xw[4]);
         } else {
-            wk1 = dz1<double>(k, nz, 5, h, [&](int kk) { return (double)by[col + (long)kk * nxy]; });
-            wk2 = dz1<double>(k, nz, 5, h, [&](int kk) { return (double)bx[col + (long)kk * nxy]; });
-            primed = false;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                wk1[c] = dz1<double>(k, nz, 5, h, [&](int kk) { return (double)by[col + c + (long)kk * nxy]; });
+                wk2[c] = dz1<double>(k, nz, 5, h, [&](int kk) { return (double)bx[col + c + (long)kk * nxy]; });
+            }
         }
         if (a.bym) {
             // mixed: bx, by hold no means; d/dz of the means (flag 5) is added back
-            wk1 += dz1<double>(k, nz, 5, h, [&](int kk) { return a.bym[kk]; });
-            wk2 += dz1<double>(k, nz, 5, h, [&](int kk) { return a.bxm[kk]; });
+            const double m1 = dz1<double>(k, nz, 5, h, [&](int kk) { return a.bym[kk]; });
+            const double m2 = dz1<double>(k, nz, 5, h, [&](int kk) { return a.bxm[kk]; });
+            wk1[0] += m1; wk1[1] += m1;
+            wk2[0] += m2; wk2[1] += m2;
         }
-        const double ex = (double)((const TW*)a.dybzh)[q];
-        const double ey = (double)((const TW*)a.dxbzh)[q];
-        const double ez = (double)((const TW*)a.jzh)[q];
-        ((TW*)a.jsq)[q] = (TW)(ez * ez + ((ex - wk1) * (ex - wk1)) + ((wk2 - ey) * (wk2 - ey)));
+        const TW2 ex = ld2(a.dybzh, q), ey = ld2(a.dxbzh, q), ez = ld2(a.jzh, q);
+        const double exv[2] = {(double)ex.x, (double)ex.y}, eyv[2] = {(double)ey.x, (double)ey.y},
+                     ezv[2] = {(double)ez.x, (double)ez.y};
+        TW2 js, jx, jy, jz;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const TW v = (TW)(ezv[c] * ezv[c] + ((exv[c] - wk1[c]) * (exv[c] - wk1[c])) +
+                              ((wk2[c] - eyv[c]) * (wk2[c] - eyv[c])));
+            const TW vx = (TW)(exv[c] - wk1[c]), vy = (TW)(wk2[c] - eyv[c]), vz = (TW)ezv[c];
+            if (c == 0) { js.x = v; jx.x = vx; jy.x = vy; jz.x = vz; }
+            else { js.y = v; jx.y = vx; jy.y = vy; jz.y = vz; }
+        }
+        *(TW2*)((TW*)a.jsq + q) = js;
         if (a.jx) {
-            ((TW*)a.jx)[q] = (TW)(ex - wk1);
-            ((TW*)a.jy)[q] = (TW)(wk2 - ey);
-            ((TW*)a.jz)[q] = (TW)ez;
+            *(TW2*)((TW*)a.jx + q) = jx;
+            *(TW2*)((TW*)a.jy + q) = jy;
+            *(TW2*)((TW*)a.jz + q) = jz;
         }
     }
 }
@@ -555,7 +581,8 @@ int launch_bandj_spec(const MagArgs& a, hipStream_t s)
 int launch_jsq(const MagArgs& a, hipStream_t s)
 {
     if (a.zc <= 0) return -10;
-    MHD_LAUNCH_W(jsq_kernel, dim3(nblk(a.g.nxy), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
+    if (a.g.nxy % 2) return -10;   // two columns per thread (nx is even)
+    MHD_LAUNCH_W(jsq_kernel, dim3(nblk(a.g.nxy / 2), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out)
