@@ -268,68 +268,107 @@ __global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
 {
     (void)av;
     const MagArgs& a = kernarg<MagArgs>();
+    using TW2 = TW __attribute__((ext_vector_type(2)));
+    using D2 = double __attribute__((ext_vector_type(2)));
     const int nx = a.g.nx, nxy = a.g.nxy, nz = a.g.nz;
-    const long col = (long)blockIdx.x * NT + threadIdx.x;
+    // a thread owns two neighbouring columns of one row (nx is even): every
+    // load and store moves two elements
+    const long col = 2 * ((long)blockIdx.x * NT + threadIdx.x);
     if (col >= nxy) return;
     const int k0 = blockIdx.y * a.zc;
     const int k1 = (k0 + a.zc < nz) ? k0 + a.zc : nz;
-    const int jq = (int)(col / nx), ip = (int)(col - (long)jq * nx);
-    const double k2 = a.akx2[ip] + a.aky2[jq];
-    const bool dc = (ip == 0 && jq == 0);
-    const bool band = in_band(ip, jq, nx, a.g.ny, a.icutx, a.icuty);
+    const int jq = (int)(col / nx), ip0 = (int)(col - (long)jq * nx);
+    double k2[2];
+    bool dc[2], band[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        k2[c] = a.akx2[ip0 + c] + a.aky2[jq];
+        dc[c] = (ip0 + c == 0 && jq == 0);
+        band[c] = in_band(ip0 + c, jq, nx, a.g.ny, a.icutx, a.icuty);
+    }
     const double* pol = a.pol;
     const double* tor = a.tor;
     const double zck = a.p.zck, s = a.p.sc1;
-    double pw[5], tw[5];
-    bool primed = false;
+    auto ld2 = [](const void* p, long q) { return *(const TW2*)((const TW*)p + q); };
+    auto ldd = [](const double* p, long q) { return *(const D2*)(p + q); };
+    // interior planes [2, nz-2) use the centred stencil from the windows
+    const int ki0 = k0 > 2 ? k0 : 2, ki1 = k1 < nz - 2 ? k1 : nz - 2;
+    double pw[2][5], tw[2][5];
+    if (ki0 < ki1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long q = col + (long)(ki0 - 2 + j) * nxy;
+            const D2 p2 = ldd(pol, q), t2 = ldd(tor, q);
+            pw[0][j + 1] = p2.x; pw[1][j + 1] = p2.y;
+            tw[0][j + 1] = t2.x; tw[1][j + 1] = t2.y;
+        }
+    }
     for (int k = k0; k < k1; ++k) {
         const long q = col + (long)k * nxy;
-        double pzz, tzz, p0, t0;
-        if (k >= 2 && k <= nz - 3) {
-            if (!primed) {
+        double pzz[2], tzz[2], p0[2], t0[2];
+        if (k >= ki0 && k < ki1) {
+            const D2 p2 = ldd(pol, q + 2L * nxy), t2 = ldd(tor, q + 2L * nxy);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    pw[j + 1] = pol[q + (long)(j - 2) * nxy];
-                    tw[j + 1] = tor[q + (long)(j - 2) * nxy];
+                    pw[c][j] = pw[c][j + 1];
+                    tw[c][j] = tw[c][j + 1];
                 }
-                primed = true;
             }
+            pw[0][4] = p2.x; pw[1][4] = p2.y;
+            tw[0][4] = t2.x; tw[1][4] = t2.y;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                pw[j] = pw[j + 1];
-                tw[j] = tw[j + 1];
+            for (int c = 0; c < 2; ++c) {
+                pzz[c] = s * (-pw[c][0] + 16.0 * pw[c][1] - 30.0 * pw[c][2] + 16.0 * pw[c][3] - pw[c][4]);
+                tzz[c] = s * (-tw[c][0] + 16.0 * tw[c][1] - 30.0 * tw[c][2] + 16.0 * tw[c][3] - tw[c][4]);
+                p0[c] = pw[c][2];
+                t0[c] = tw[c][2];
             }
-            pw[4] = pol[q + 2L * nxy];
-            tw[4] = tor[q + 2L * nxy];
-            pzz = s * (-pw[0] + 16.0 * pw[1] - 30.0 * pw[2] + 16.0 * pw[3] - pw[4]);
-            tzz = s * (-tw[0] + 16.0 * tw[1] - 30.0 * tw[2] + 16.0 * tw[3] - tw[4]);
-            p0 = pw[2];
-            t0 = tw[2];
         } else {
-            pzz = dz2<double>(k, nz, 3, s, [&](int kk) { return pol[col + (long)kk * nxy]; });
-            tzz = dz2<double>(k, nz, 3, s, [&](int kk) { return tor[col + (long)kk * nxy]; });
-            p0 = pol[q];
-            t0 = tor[q];
-            primed = false;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                pzz[c] = dz2<double>(k, nz, 3, s, [&](int kk) { return pol[col + c + (long)kk * nxy]; });
+                tzz[c] = dz2<double>(k, nz, 3, s, [&](int kk) { return tor[col + c + (long)kk * nxy]; });
+                p0[c] = pol[q + c];
+                t0[c] = tor[q + c];
+            }
         }
-        const double il_c = dc ? 0.0 : (double)((const TW*)a.cc)[q] / k2;
-        const double il_d = dc ? 0.0 : (double)((const TW*)a.dz)[q] / k2;
-        const TW pd = (TW)(zck * p0 * k2 + zck * pzz + il_c);
-        const TW td = (TW)(zck * t0 * k2 + zck * tzz - il_d + (double)((const TW*)a.ezh)[q]);
-        ((TW*)a.poldot)[q] = pd;
-        ((TW*)a.tordot)[q] = td;
+        const TW2 cc2 = ld2(a.cc, q), dz2v = ld2(a.dz, q), ez2 = ld2(a.ezh, q);
+        const double ccv[2] = {(double)cc2.x, (double)cc2.y}, dzv[2] = {(double)dz2v.x, (double)dz2v.y},
+                     ezv[2] = {(double)ez2.x, (double)ez2.y};
+        TW pd[2], td[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double il_c = dc[c] ? 0.0 : ccv[c] / k2[c];
+            const double il_d = dc[c] ? 0.0 : dzv[c] / k2[c];
+            pd[c] = (TW)(zck * p0[c] * k2[c] + zck * pzz[c] + il_c);
+            td[c] = (TW)(zck * t0[c] * k2[c] + zck * tzz[c] - il_d + ezv[c]);
+        }
+        TW2 pdv, tdv;
+        pdv.x = pd[0]; pdv.y = pd[1];
+        tdv.x = td[0]; tdv.y = td[1];
+        *(TW2*)((TW*)a.poldot + q) = pdv;
+        *(TW2*)((TW*)a.tordot + q) = tdv;
         if (a.ab3) {
             // x + c0 f3 + c1 f2 + c2 f1, then the 2/3 band of pol and tor cleared
-            double pn = p0 + a.c0 * (double)pd + a.c1 * (double)((const TW*)a.ph2)[q] +
-                        a.c2 * (double)((const TW*)a.ph1)[q];
-            double tn = t0 + a.c0 * (double)td + a.c1 * (double)((const TW*)a.th2)[q] +
-                        a.c2 * (double)((const TW*)a.th1)[q];
-            if (band) {
-                pn = 0.0;
-                tn = 0.0;
+            const TW2 ph2 = ld2(a.ph2, q), ph1 = ld2(a.ph1, q), th2 = ld2(a.th2, q), th1 = ld2(a.th1, q);
+            const double ph2v[2] = {(double)ph2.x, (double)ph2.y}, ph1v[2] = {(double)ph1.x, (double)ph1.y};
+            const double th2v[2] = {(double)th2.x, (double)th2.y}, th1v[2] = {(double)th1.x, (double)th1.y};
+            D2 pn, tn;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double pv = p0[c] + a.c0 * (double)pd[c] + a.c1 * ph2v[c] + a.c2 * ph1v[c];
+                double tv = t0[c] + a.c0 * (double)td[c] + a.c1 * th2v[c] + a.c2 * th1v[c];
+                if (band[c]) {
+                    pv = 0.0;
+                    tv = 0.0;
+                }
+                if (c == 0) { pn.x = pv; tn.x = tv; }
+                else { pn.y = pv; tn.y = tv; }
             }
-            a.polnew[q] = pn;
-            a.tornew[q] = tn;
+            *(D2*)(a.polnew + q) = pn;
+            *(D2*)(a.tornew + q) = tn;
         }
     }
 }
@@ -569,7 +608,8 @@ int launch_mean_emf(const MagArgs& a, hipStream_t s)
 int launch_mag_rhs(const MagArgs& a, hipStream_t s)
 {
     if (a.zc <= 0) return -10;
-    MHD_LAUNCH_W(mag_rhs_kernel, dim3(nblk(a.g.nxy), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
+    if (a.g.nxy % 2) return -10;   // two columns per thread (nx is even)
+    MHD_LAUNCH_W(mag_rhs_kernel, dim3(nblk(a.g.nxy / 2), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_bandj_spec(const MagArgs& a, hipStream_t s)
