@@ -513,11 +513,29 @@ static int launch_mode(const SpecLaunch& L, hipStream_t s)
     return -23;
 }
 
+// the single-output derivative passes along y (one input, one output per
+// line) with half the column pairs per workgroup (512 threads instead of 1024
+// at 256 points: two workgroups per CU fit where one did)
+template <typename TC, typename PL, int B>
+static int launch_y_deriv1(const SpecLaunch& L, hipStream_t s)
+{
+    switch (L.job[0].load_op) {
+    case LD_PLAIN: return launch_one<TC, PL, 1, B, SM_DERIV1, LD_PLAIN>(L, s);
+    case LD_PROD: return launch_one<TC, PL, 1, B, SM_DERIV1, LD_PROD>(L, s);
+    case LD_MAXW: return launch_one<TC, PL, 1, B, SM_DERIV1, LD_MAXW>(L, s);
+    }
+    return launch_mode<TC, PL, 1, B>(L, s);
+}
+
 template <typename TC, typename PL>
 static int launch_plan(const SpecLaunch& L, hipStream_t s)
 {
     if (L.dir == 0) return launch_mode<TC, PL, 0, spec_b<TC, PL, 0>()>(L, s);
     constexpr int B0 = spec_b<TC, PL, 1>();
+    // FP32 only: at 256 points 8 pairs beat 16 for these passes (76, 72, 45 us
+    // against 81, 83, 48); in FP64 4 pairs lose to 8
+    constexpr int BH = (sizeof(TC) == 4 && B0 >= 8) ? B0 / 2 : B0;
+    if (BH != B0 && L.mode == SM_DERIV1 && L.g.nx % (2 * BH) == 0) return launch_y_deriv1<TC, PL, BH>(L, s);
     if (L.g.nx % (2 * B0) == 0) return launch_mode<TC, PL, 1, B0>(L, s);
     return launch_mode<TC, PL, 1, 1>(L, s);
 }
