@@ -102,6 +102,24 @@ def test_diagnostics(pair):
     assert rel(s.field("u"), o["u"]) < 1e-12
 
 
+def test_diagnostics_tall_box():
+    # nz = 520: the reductions' per-plane partials (8 values a plane) exceed the
+    # finishing workgroup's LDS and are combined in device memory instead
+    print("[physics] diagnostics 8x8x520", flush=True)
+    s, o = make_pair(8, 8, 520, seed=5, b_amp=1e-3)
+    try:
+        assert abs(s.nusselt() - o.nusselt()) < 1e-12
+        db_s, db_o = s.checkdivb(), o.checkdivb()
+        assert abs(db_s[0] - db_o[0]) < 1e-10 and abs(db_s[1] - db_o[1]) < 1e-10
+        for n in ("t", "u"):
+            assert np.allclose(s.sminmax(n), o.sminmax(o[n]), rtol=1e-14, atol=0)
+        assert abs(s.calcdt() - o.calcdt()) < 1e-15 * o.calcdt() * 10
+        ga_s, ga_o = np.array(s.GlobalAverages()), o.global_averages()
+        assert np.allclose(ga_s, ga_o, rtol=1e-11, atol=1e-18)
+    finally:
+        s.close()
+
+
 @pytest.mark.parametrize("box,steps", [((16, 16, 12), 30), ((64, 64, 48), 12)],
                          ids=["16x16x12-30", "64x64x48-12"])
 def test_steps_fixed_dt(box, steps):
