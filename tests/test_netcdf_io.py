@@ -168,3 +168,17 @@ def test_errors(tmp_path):
         fa.read_dump(tmp_path / "other.nc")
     with pytest.raises(fa.MhdError):
         fa.write_dump(tmp_path / "x.nc", _fields((2, 2, 2)), iprec=3)
+
+
+def test_truncated_files_are_errors(tmp_path):
+    # every cut of a valid dump (inside the header, inside a variable) is
+    # reported as an unreadable file, never read past its end
+    good = tmp_path / "good.nc"
+    fa.write_dump(good, _fields((4, 6, 8)), iteration=3, history="truncation test")
+    data = good.read_bytes()
+    for cut in (8, 40, 200, len(data) // 2, len(data) - 8, len(data) - 1):
+        bad = tmp_path / f"cut{cut}.nc"
+        bad.write_bytes(data[:cut])
+        with pytest.raises(fa.MhdError):
+            fa.read_dump(bad)
+
