@@ -14,9 +14,13 @@ state, means, backgrounds and reductions); its rate, its speed-up and its
 relative error against the FP64 run (fields, Nu, kinetic and magnetic
 energy) are reported under "mixed".
 
-Prints one JSON line.  The roofline object covers the dominant kernel family
-(the x-direction spectral first+second derivative of T, u, v, w), timed with
-HIP events on the solver's stream over the timed steps.  step_roofline puts
+Prints one JSON line.  The roofline object covers the dominant kernel family,
+hydro_col (the whole hydrodynamic right-hand side after the x/y transform
+passes, with the fused AB3 update of u, v, w, rho, T: 55 arrays per cell in
+FP64, 300 bytes per cell in the mixed mode), timed with HIP events on the
+solver's stream over the timed steps; roofline_xderiv2 does the same for the
+x-direction first+second derivative pass of T, u, v, w over a few further
+steps.  step_roofline puts
 the whole step against the same peak at its algorithmic bytes per cell-step
 (139 working-precision array passes, the FP64 state read and written, the RHS
 history: 1448 B in FP64, 780 B mixed).  cpu_baseline times
@@ -79,16 +83,20 @@ def cpu_baseline(params, steps=3, steps1=1):
     pyoracle.set_threads(nthreads)
     cells = NX * NY * NZ
     return {"value": round(cells * steps / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": nthreads,
-            "kind": "port",
+            "kind": "port", "host_logical_cpus": os.cpu_count(),
             "single_thread": round(cells * steps1 / dt1 / 1e6, 3),
             "sample": f"CPU oracle (C restatement of the reference, FP64), full {NX}x{NY}x{NZ} box: "
                       f"{steps} AB3 steps on {nthreads} threads ({dt:.1f} s) and {steps1} on 1 thread "
                       f"({dt1:.1f} s), each after 1 untimed step"}
 
 
-def run(precision, steps, warmup, kernel, wl):
-    """Static start, `warmup` untimed steps, `steps` timed steps; returns the
-    live solver with the timings."""
+SECOND_KERNEL_STEPS = 5
+
+
+def run(precision, steps, warmup, kernel, wl, second="xderiv2"):
+    """Static start, `warmup` untimed steps, `steps` timed steps (kernel family
+    `kernel` timed with events), then SECOND_KERNEL_STEPS untimed steps with
+    the family `second` timed; returns the live solver with the timings."""
     from fug30v0kxc_amd import MHD_PREC_FP64, MHD_PREC_MIXED, Solver
     nx, ny, nz, dt, extra, _ = wl
     prec = MHD_PREC_FP64 if precision == "fp64" else MHD_PREC_MIXED
@@ -105,21 +113,32 @@ def run(precision, steps, warmup, kernel, wl):
     t1 = time.perf_counter()
     kms, klaunch = s.kernel_time()
     s.kernel_timing(kernel, False)
-    return {"solver": s, "elapsed": t1 - t0, "diag": diag, "kms": kms, "klaunch": klaunch, "tune": tune,
-            "cells": nx * ny * nz, "device_bytes": s.device_bytes(), "done": warmup + steps}
+    out = {"solver": s, "elapsed": t1 - t0, "diag": diag, "kms": kms, "klaunch": klaunch, "tune": tune,
+           "cells": nx * ny * nz, "device_bytes": s.device_bytes(), "done": warmup + steps}
+    if second and second != kernel:
+        s.kernel_timing(second, True)
+        s.step(SECOND_KERNEL_STEPS, dt, NDU06)
+        s.sync()
+        out["kms2"], out["klaunch2"] = s.kernel_time()
+        s.kernel_timing(second, False)
+        out["done"] += SECOND_KERNEL_STEPS
+    return out
 
 
-def ensemble_bench(steps, seed=20261202):
-    """32 seeded independent boxes (nx = ny in {64, 96, 128}, nz in {48, 64, 96},
-    Ra and Q log-uniform in [1e4, 1e6] and [10, 1e3], eta(z) contrast
-    log-uniform in [1, 100] at z = 0.5), mixed precision, each autotuned;
-    `steps` steps of all boxes by hipGraph replays against the same boxes
-    stepped one after another."""
+C4_SEED = 20261202
+C4_DT = 1e-6
+
+
+def c4_specs(seed=C4_SEED, nbox=32):
+    """The ensemble workload's box table: (n, nz, params) of `nbox` seeded
+    independent boxes, nx = ny in {64, 96, 128}, nz in {48, 64, 96}, Ra and Q
+    log-uniform in [1e4, 1e6] and [10, 1e3], eta(z) contrast log-uniform in
+    [1, 100] at z = 0.5 (width 0.05), T and Bz noise 1e-3."""
     import numpy as np
-    from fug30v0kxc_amd import MHD_PREC_MIXED, Ensemble, Solver, reference_params
+    from fug30v0kxc_amd import reference_params
     rng = np.random.default_rng(seed)
     specs = []
-    for _ in range(32):
+    for _ in range(nbox):
         n = int(rng.choice([64, 96, 128]))
         nz = int(rng.choice([48, 64, 96]))
         ra = 10 ** rng.uniform(4, 6)
@@ -128,7 +147,17 @@ def ensemble_bench(steps, seed=20261202):
         p = reference_params(ra=ra, q=q, t_amp=1e-3, b_amp=1e-3)
         p.update(eta_c=ec, eta_z0=0.5, eta_w=0.05)
         specs.append((n, nz, p))
-    dt = 1e-6
+    return specs
+
+
+def ensemble_bench(steps, seed=C4_SEED):
+    """32 seeded independent boxes (c4_specs), mixed precision, each autotuned;
+    `steps` steps of all boxes by hipGraph replays against the same boxes
+    stepped one after another."""
+    import numpy as np
+    from fug30v0kxc_amd import MHD_PREC_MIXED, Ensemble, Solver
+    specs = c4_specs(seed)
+    dt = C4_DT
     cells = sum(n * n * nz for n, nz, _ in specs)
 
     def make(tunes=None):
@@ -172,29 +201,30 @@ def ensemble_bench(steps, seed=20261202):
             "capture_s": round(t_capture, 2), "identical_to_sequential": bool(same)}
 
 
-def closure_bench(nfine, seed=20261203):
-    """Learned closure: a fine run (nfine^2 x 3 nfine / 4, Ra 1e6, Q 500, T and
-    Bz noise 0.1 and 0.01, mixed precision) is box-filtered (factor 4) into
-    coarse snapshots; the stencil MLP is trained on the GPU on the earlier
-    ones.  Reported: the fraction of subgrid-flux variance it explains on a
-    later, held-out snapshot (the plain coarse method has no subgrid term and
-    scores 0); coarse runs with and without the closure from the filtered
-    held-out state against the filtered fine run (kinetic-energy spectra, Nu)
-    and the fallback decision; cost per unit of simulated time; the forward
-    and training kernels' rates."""
+def closure_bench(nfine, f=4, ra=1e6, seed=20261203):
+    """Learned closure: a fine run (nfine^2 x 3 nfine / 4, Ra `ra`, Q 500, T
+    and Bz noise 0.1 and 0.01, mixed precision) is coarse-grained (factor f
+    horizontally, onto the coarse vertices in z) into snapshots straight from
+    its device arrays; the stencil MLP is trained on the GPU on the earlier
+    ones.  Reported: the centred fraction of subgrid-flux variance it explains
+    on a later, held-out snapshot (the plain coarse method has no subgrid term
+    and scores 0); coarse runs with and without the closure from the
+    filtered held-out state against the filtered fine run (kinetic-energy
+    spectra, and Nu of the filtered fine state on the coarse grid) and the
+    fallback decision; cost per unit of simulated time; the data path's and
+    the forward and training kernels' rates."""
     import numpy as np
     from fug30v0kxc_amd import MHD_PREC_MIXED, Closure, Solver, closure_targets, reference_params
     from fug30v0kxc_amd import closure_eval as ce
-    f = 4
     fine_box = (nfine, nfine, 3 * nfine // 4)
     coarse_box = tuple(n // f for n in fine_box)
-    p = reference_params(ra=1e6, q=500.0, t_amp=0.1, b_amp=0.01)
+    p = reference_params(ra=ra, q=500.0, t_amp=0.1, b_amp=0.01)
     fine = Solver(*fine_box, p, precision=MHD_PREC_MIXED)
     fine.static()
     blk = 500
 
     def note(msg):
-        print(f"[c5 {nfine}] {msg}", file=sys.stderr, flush=True)
+        print(f"[c5 {nfine}/{f}] {msg}", file=sys.stderr, flush=True)
 
     # spin-up with the reference's calcdt until t = 0.25
     nspin = 0
@@ -204,40 +234,49 @@ def closure_bench(nfine, seed=20261203):
         sc = fine.scalars()
         note(f"spin-up: {nspin} steps, t = {sc['ttime']:.4f}, dt = {sc['dt']:.3e}")
     dtf = fine.scalars()["dt"]
-    snaps, nu_f = [], []
-    t0 = time.perf_counter()
-    for _ in range(6):
-        d = fine.step(blk, dtf, blk)
-        snaps.append(closure_targets(fine.get_state(), f, p["f"]))
-        nu_f.append(d["nusselt"])
-        note(f"snapshot {len(snaps)} of 6")
-    fine.sync()
-    fine_ms_per_step = None
     cl = Closure(seed=1)
-    for c, t in snaps[:5]:
-        cl.add(c, t)
+    add_ms = []
+    for i in range(5):
+        fine.step(blk, dtf, blk)
+        fine.sync()
+        t0 = time.perf_counter()
+        cl.add_solver(fine, f)          # device-resident coarse-graining into the data set
+        add_ms.append((time.perf_counter() - t0) * 1e3)
+        note(f"snapshot {i + 1} of 5")
+    fine.step(blk, dtf, blk)
+    held = closure_targets(fine.get_state(), f, p["f"])
     t1 = time.perf_counter()
     loss = cl.train(6000, batch=4096, lr=1e-3)
     train_s = time.perf_counter() - t1
     note(f"trained: 6000 steps in {train_s:.2f} s, last batch loss {loss:.4f}")
-    r2 = ce.explained_variance(cl.predict(snaps[5][0]), snaps[5][1])
+    r2 = ce.explained_variance(cl.predict(held[0]), held[1])
     fwd_ms, train_ms, nfwd = cl.time(batch=8192, reps=20)
     flops_cell = 2 * (216 * 64 + 64 * 64 + 64 * 12)
+
+    # Nu of a filtered fine state on the coarse grid
+    probe = Solver(*coarse_box, p, precision=MHD_PREC_MIXED)
+
+    def nu_filtered(coarse_state):
+        probe.set_state(*[coarse_state[n] for n in ("r", "t", "u", "v", "w", "bx", "by", "bz")])
+        return probe.nusselt()
+
     # a posteriori: 4 blocks from the held-out state
     ref = []
     t2 = time.perf_counter()
     for _ in range(4):
         d = fine.step(blk, dtf, blk)
+        fine.sync()
         ref.append((closure_targets(fine.get_state(), f, p["f"])[0], d["nusselt"]))
         note(f"reference block {len(ref)} of 4")
-    fine.sync()
     fine_ms_per_step = (time.perf_counter() - t2) / (4 * blk) * 1e3
     fine.close()
+    nu_ref = [nu_filtered(st) for st, _ in ref]
+    probe.close()
     runs = {}
     for mode in ("plain", "closure"):
         co = Solver(*coarse_box, p, precision=MHD_PREC_MIXED)
         co.static()
-        c0 = snaps[5][0]
+        c0 = held[0]
         co.set_state(*[c0[n] for n in ("r", "t", "u", "v", "w", "bx", "by", "bz")])
         if mode == "closure":
             co.attach_closure(cl)
@@ -254,20 +293,28 @@ def closure_bench(nfine, seed=20261203):
             nus.append(d["nusselt"])
         co.sync()
         ms = (time.perf_counter() - tc) / (nsteps - 2) * 1e3
+        nu_err = float(np.mean([ce.nusselt_error(a, b) for a, b in zip(nus, nu_ref)]))
         runs[mode] = {"spectral_error": [round(e, 4) for e in errs], "mean_spectral_error": round(float(np.mean(errs)), 4),
-                      "nusselt": [round(x, 4) for x in nus], "ms_per_step": round(ms, 4)}
-        note(f"coarse {mode}: mean spectral error {runs[mode]['mean_spectral_error']}, {ms:.4f} ms/step")
+                      "nusselt": [round(x, 4) for x in nus], "mean_nusselt_error": round(nu_err, 4),
+                      "ms_per_step": round(ms, 4)}
+        note(f"coarse {mode}: mean spectral error {runs[mode]['mean_spectral_error']}, "
+             f"Nu error {runs[mode]['mean_nusselt_error']}, {ms:.4f} ms/step")
         co.close()
     cost_fine = fine_ms_per_step * f                 # one coarse step's simulated time
     return {
-        "fine_box": list(fine_box), "coarse_box": list(coarse_box), "filter": f, "dt_fine": dtf,
+        "fine_box": list(fine_box), "coarse_box": list(coarse_box), "filter": f, "ra": ra, "dt_fine": dtf,
+        "data_path": {"add_snapshot_ms": [round(x, 2) for x in add_ms], "fine_cells": int(np.prod(fine_box)),
+                      "coarse_cells": int(np.prod(coarse_box))},
         "training": {"snapshots": 5, "samples": cl.samples, "steps": 6000, "batch": 4096, "seconds": round(train_s, 3),
                      "last_batch_loss": round(loss, 4)},
         "a_priori_explained_variance": {"closure": [round(float(x), 3) for x in r2], "closure_mean": round(float(np.mean(r2)), 3),
-                                        "plain": 0.0},
+                                        "plain": 0.0, "definition": "centred R^2 per flux, held-out snapshot"},
         "a_posteriori": {**runs, "fine_nusselt": [round(x[1], 4) for x in ref],
-                         "decision": ce.choose(runs["closure"]["mean_spectral_error"]),
-                         "fallback_threshold": ce.FALLBACK_SPECTRAL_ERROR},
+                         "filtered_fine_nusselt_on_coarse_grid": [round(x, 4) for x in nu_ref],
+                         "decision": ce.choose(runs["closure"]["mean_spectral_error"],
+                                               nu_error=runs["closure"]["mean_nusselt_error"]),
+                         "fallback_threshold": ce.FALLBACK_SPECTRAL_ERROR,
+                         "fallback_nusselt_threshold": ce.FALLBACK_NUSSELT_ERROR},
         "cost_ratio_fine_over_closure_coarse": round(cost_fine / runs["closure"]["ms_per_step"], 1),
         "mlp_forward": {"cells": nfwd, "ms": round(fwd_ms, 4),
                         "tflops": round(nfwd * flops_cell / (fwd_ms * 1e-3) / 1e12, 1), "peak_tflops_bf16": 2500.0},
@@ -398,16 +445,32 @@ def advance(r, total, dt):
 
 
 def measured_traffic(precision, kernel):
-    """HBM bytes per xderiv2 launch from the committed FETCH_SIZE / WRITE_SIZE
-    summary (profiles/hbm_bytes_256x256x160.md), or None."""
+    """HBM bytes per launch of a kernel family from the committed FETCH_SIZE /
+    WRITE_SIZE summary (profiles/hbm_bytes_256x256x160.md, lines
+    `traffic_bytes_per_launch <kernel> <precision> <bytes>`), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_bytes_256x256x160.md")
-    if kernel != "xderiv2" or not os.path.exists(path):
+    if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == precision:
-            return int(f[2])
+        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == precision:
+            return int(f[3])
     return None
+
+
+# algorithmic bytes per cell of one launch of each timed kernel family
+#  xderiv2: 4 fields, 1 read + 2 writes each, working precision
+#  hydro:   hydro_col with the fused AB3 update (interior and wall-row launches):
+#           reads the state (5), B and J^2 (4), the 26 x/y derivative arrays and
+#           history levels 1, 2 of 5 fields; writes 5 RHS and the 5 new state
+#           fields.  FP64: 55 x 8 = 440 B.  Mixed: FP32 state shadows, B, J^2,
+#           derivatives and history, plus the FP64 state read and written and
+#           the new FP32 shadows: 4 (5 + 4 + 26 + 10 + 5 + 5) + 8 (5 + 5) = 300 B.
+KERNEL_BYTES_PER_CELL = {"xderiv2": {8: 96, 4: 48}, "yderiv2": {8: 96, 4: 48}, "hydro": {8: 440, 4: 300}}
+KERNEL_DESC = {"xderiv2": "spec_kernel xderiv2 (d/dx and d2/dx2 of T, u, v, w)",
+               "yderiv2": "spec_kernel yderiv2 (d/dy and d2/dy2 of T, u, v, w)",
+               "hydro": "hydro_col_kernel (interior + wall rows): hydro_deriv's right-hand side after the "
+                        "x/y passes with the fused AB3 update of u, v, w, rho, T"}
 
 
 # algorithmic bytes per cell-step of the whole step: 139 working-precision array
@@ -425,18 +488,17 @@ def step_roofline(cells, ms_per_step, elem):
             "ceiling_mcell_steps_per_s": round(HBM_PEAK_GBS * 1e9 / b / 1e6, 1)}
 
 
-def roofline_of(r, kernel, elem, workload="c2"):
+def roofline_of(r, kernel, elem, workload="c2", second=False):
     cells = r["cells"]
-    # algorithmic bytes of one launch: 4 fields, 1 read + 2 writes each
-    bytes_per_launch = 4 * 3 * elem * cells
-    avg_ms = r["kms"] / max(r["klaunch"], 1)
+    bytes_per_launch = KERNEL_BYTES_PER_CELL[kernel][elem] * cells
+    kms, nl = (r["kms2"], r["klaunch2"]) if second else (r["kms"], r["klaunch"])
+    avg_ms = kms / max(nl, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic("fp64" if elem == 8 else "mixed", kernel) if workload == "c2" else None
+    traffic = measured_traffic(kernel, "fp64" if elem == 8 else "mixed") if workload == "c2" else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": f"spec_kernel {kernel} (4 fields: d/dx and d2/dx2 of T,u,v,w)",
-            "avg_launch_ms": round(avg_ms, 4), "launches": r["klaunch"],
+            "traffic": traffic, "kernel": KERNEL_DESC[kernel],
+            "avg_launch_ms": round(avg_ms, 4), "launches": nl,
             "bytes_per_launch": bytes_per_launch}
 
 
@@ -449,10 +511,12 @@ def main():
     ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"),
                     help="both: FP64 headline plus the mixed run; fp64 / mixed: that run alone "
                          "(e.g. for a kernel profile of one mode)")
-    ap.add_argument("--kernel", default="xderiv2", choices=("xderiv2", "yderiv2"),
-                    help="kernel family for the roofline object")
+    ap.add_argument("--kernel", default="hydro", choices=("hydro", "xderiv2", "yderiv2"),
+                    help="kernel family for the roofline object (xderiv2 is also reported beside it)")
     ap.add_argument("--c5-fine", type=int, default=256,
                     help="c5 (closure) workload: fine nx = ny (nz = 3/4 of it); 512 is the 512^2 x 384 case")
+    ap.add_argument("--c5-factor", type=int, default=4, help="c5: coarse-graining factor")
+    ap.add_argument("--c5-ra", type=float, default=1e6, help="c5: Rayleigh number")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "aux"],
                     help="c2: 256x256x160 (default); c3: 512x512x256 with the eta(z) step; "
                          "c4: ensemble of 32 boxes in hipGraph replays; c5: learned closure; "
@@ -507,13 +571,14 @@ def main():
         print(json.dumps(out))
         return 0
     if args.workload == "c5":
-        r = closure_bench(args.c5_fine)
+        r = closure_bench(args.c5_fine, args.c5_factor, args.c5_ra)
         out = {"metric": "closure: explained subgrid-flux variance (held out) and coarse/fine spectral error",
                "value": r["a_priori_explained_variance"]["closure_mean"], "unit": "fraction", "n_gpus": 1,
                "steps": None, "warmup": None, "ms_per_step": r["a_posteriori"]["closure"]["ms_per_step"],
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                "dtype": "bf16 (FP32 accumulate); mixed-precision solver", "data": "synthetic",
-               "config": {"workload": "learned closure, fine run box-filtered by 4, Ra 1e6, Q 500", **r},
+               "config": {"workload": f"learned closure, fine run coarse-grained by {args.c5_factor}, "
+                                      f"Ra {args.c5_ra:g}, Q 500", **r},
                "roofline": {"bound": "mfma", "achieved": r["mlp_forward"]["tflops"], "peak": 2500.0,
                             "unit": "TFLOP/s", "frac": round(r["mlp_forward"]["tflops"] / 2500.0, 4),
                             "traffic": None},
@@ -536,6 +601,8 @@ def main():
                "roofline": roofline_of(rm, args.kernel, 4, args.workload), "cpu_baseline": None,
                "step_roofline": step_roofline(cells, rm["elapsed"] / args.steps * 1e3, 4),
                "device_bytes": rm["device_bytes"]}
+        if "kms2" in rm:
+            out["roofline_xderiv2"] = roofline_of(rm, "xderiv2", 4, args.workload, second=True)
         rm["solver"].close()
         print(json.dumps(out))
         return 0
@@ -566,6 +633,8 @@ def main():
                                         for k, v in qe.items()},
                                      "fields": {k: float(f"{v:.3e}") for k, v in fe.items()}},
                  "roofline": roofline_of(rm, args.kernel, 4, args.workload),
+                 "roofline_xderiv2": (roofline_of(rm, "xderiv2", 4, args.workload, second=True)
+                                      if "kms2" in rm else None),
                  "step_roofline": step_roofline(cells, rm["elapsed"] / args.steps * 1e3, 4),
                  "tune": rm.get("tune"),
                  "device_bytes": rm["device_bytes"]}
@@ -586,12 +655,17 @@ def main():
                       "nx": nx, "ny": ny, "nz": nz, "dt": dt, "ndu06": NDU06,
                       "precision": "fp64"},
            "roofline": roofline_of(r64, args.kernel, 8, args.workload), "cpu_baseline": cpu,
+           "roofline_xderiv2": (roofline_of(r64, "xderiv2", 8, args.workload, second=True)
+                                if "kms2" in r64 else None),
            "step_roofline": step_roofline(cells, r64["elapsed"] / args.steps * 1e3, 8),
            "mixed": mixed,
            "tune": r64.get("tune"),
            "device_bytes": r64["device_bytes"],
-           "diag": {"nusselt": d64.get("nusselt"), "kinenergy": d64.get("kinenergy"),
-                    "magenergy": d64.get("magenergy"), "finite": finite}}
+           # the diagnostics of the last ndu06 step inside the timed steps (null when
+           # none fell there), and whether the final state is finite
+           "diag": ({"nit": d64.get("nit"), "nusselt": d64.get("nusselt"), "kinenergy": d64.get("kinenergy"),
+                     "magenergy": d64.get("magenergy")} if d64.get("nit", -1) >= 0 else None),
+           "state_finite": finite}
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(out))

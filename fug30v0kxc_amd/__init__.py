@@ -351,10 +351,12 @@ class Solver:
 
 class Ensemble:
     """Independent boxes (Solver objects on this GPU) stepped together: the
-    kernels of `block` steps of every box are captured once into one
-    hipGraph (each box on its own stream branch) and replayed.  Results
-    equal Solver.step(n, dt) per box.  A box must not be stepped on its own
-    while it belongs to an ensemble."""
+    kernel launches of `block` steps of every box are recorded, the k-th
+    launches of the boxes of one class (same lengths, precision, options)
+    become one launch over a table of boxes, and those batched launches are
+    captured once into one hipGraph and replayed.  Results equal
+    Solver.step(n, dt) per box, bit for bit.  A box must not be stepped on
+    its own while it belongs to an ensemble."""
 
     def __init__(self, solvers, dt):
         self._lib = _abi.load()
@@ -372,6 +374,10 @@ class Ensemble:
     @property
     def nodes(self):
         return self._lib.mhd_ensemble_nodes(self._e)
+
+    @property
+    def classes(self):
+        return self._lib.mhd_ensemble_classes(self._e)
 
     def step(self, nsteps):
         code = self._lib.mhd_ensemble_step(self._e, int(nsteps))
@@ -497,6 +503,11 @@ class Closure:
         coarse, tgt = closure_targets(fine, factor, fmag)
         self.add(coarse, tgt)
         return coarse, tgt
+
+    def add_solver(self, solver, factor):
+        """Filter a fine Solver's current state on the device and append it
+        (no host round trip of the fields)."""
+        self._chk(self._lib.mhd_closure_add_state(self._c, solver._ctx, int(factor)), "add_solver")
 
     @property
     def samples(self):

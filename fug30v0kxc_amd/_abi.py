@@ -131,8 +131,11 @@ def load():
         "mhd_ensemble_step": (i, [vp, i]),
         "mhd_ensemble_block": (i, []),
         "mhd_ensemble_nodes": (ctypes.c_long, [vp]),
+        "mhd_ensemble_classes": (i, [vp]),
         "mhd_ensemble_destroy": (None, [vp]),
         "mhd_expand": (i, [i, i, i, dp, dp]),
+        "mhd_dump_header": (ctypes.c_long, [ctypes.POINTER(MhdDump), i, ctypes.c_char_p, ctypes.c_long,
+                                            ctypes.POINTER(i)]),
         "mhd_box_filter": (i, [i, i, i, i, dp, dp]),
         "mhd_closure_nparam": (i, []),
         "mhd_closure_create": (i, [ctypes.c_uint, ctypes.POINTER(vp)]),
@@ -141,6 +144,7 @@ def load():
         "mhd_closure_get_params": (i, [vp, fp, i]),
         "mhd_closure_targets": (i, [i, i, i, i, d, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp)]),
         "mhd_closure_add": (i, [vp, i, i, i, ctypes.POINTER(dp), ctypes.POINTER(dp)]),
+        "mhd_closure_add_state": (i, [vp, vp, i]),
         "mhd_closure_samples": (i, [vp, ctypes.POINTER(ctypes.c_long)]),
         "mhd_closure_get_norm": (i, [vp, dp]),
         "mhd_closure_set_norm": (i, [vp, dp]),

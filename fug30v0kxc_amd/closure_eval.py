@@ -4,12 +4,14 @@
 The measures are the horizontal kinetic-energy spectrum (shells of integer
 |k| on the coarse grid, averaged over z), its relative L1 distance up to the
 2/3 cutoff, the a-priori fraction of subgrid-flux variance a prediction
-explains, and the rule that falls back to the plain coarse run when the
-coupled run's spectral error exceeds a threshold.
+explains (the centred coefficient of determination), and the rule that
+falls back to the plain coarse run when the coupled run's spectral error or
+its Nusselt number strays beyond a threshold from the filtered fine run.
 """
 import numpy as np
 
 FALLBACK_SPECTRAL_ERROR = 0.10
+FALLBACK_NUSSELT_ERROR = 0.10
 
 
 def energy_spectrum(state, kmax=None):
@@ -38,14 +40,27 @@ def spectral_error(state, reference, kmax=None):
 
 
 def explained_variance(prediction, target):
-    """Per-output 1 - <(p - t)^2> / <t^2>: the fraction of the subgrid flux the
-    prediction accounts for; the plain coarse method (no subgrid term) scores 0."""
+    """Per-output centred R^2 = 1 - sum (p - t)^2 / sum (t - mean t)^2: the
+    fraction of the subgrid flux's variance the prediction accounts for.
+    Predicting the mean scores 0, as does the plain coarse method."""
     p = np.asarray(prediction).reshape(len(prediction), -1)
     t = np.asarray(target).reshape(len(target), -1)
-    return 1.0 - ((p - t) ** 2).mean(axis=1) / (t ** 2).mean(axis=1)
+    tc = t - t.mean(axis=1, keepdims=True)
+    return 1.0 - ((p - t) ** 2).sum(axis=1) / (tc ** 2).sum(axis=1)
 
 
-def choose(closure_error, threshold=FALLBACK_SPECTRAL_ERROR):
+def nusselt_error(nu, nu_ref):
+    """|Nu - Nu_ref| / Nu_ref."""
+    return abs(nu - nu_ref) / abs(nu_ref)
+
+
+def choose(closure_error, threshold=FALLBACK_SPECTRAL_ERROR, nu_error=None,
+           nu_threshold=FALLBACK_NUSSELT_ERROR):
     """'closure' while the coupled run stays within the threshold of the
-    filtered fine spectra, else 'plain' (the fallback)."""
-    return "closure" if closure_error <= threshold else "plain"
+    filtered fine spectra and, when given, its Nusselt number within
+    nu_threshold of the filtered fine run's; else 'plain' (the fallback)."""
+    if closure_error > threshold:
+        return "plain"
+    if nu_error is not None and nu_error > nu_threshold:
+        return "plain"
+    return "closure"

@@ -61,6 +61,18 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 __device__ __forceinline__ bf16x8 frag(const __bf16* p) { return *(const bf16x8*)p; }
+// A normalised FP64 feature to bf16 in two explicit roundings, each to
+// nearest even: FP64 -> FP32, then FP32 -> bf16 on the bits.  The second
+// rounding is written out so that a value whose FP32 rounding lands exactly
+// on a bf16 tie is resolved the same way everywhere (a direct FP64 -> bf16
+// rounding would differ there).
+__device__ __forceinline__ __bf16 feature_bf16(double x)
+{
+    const float f = (float)x;
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
+}
 // tanh-form GELU: g(z) = z (1 + tanh(k (z + a z^3))) / 2, k = sqrt(2/pi), a = 0.044715.
 // With s = 1 / (1 + e^{-2k(z + a z^3)}) (so (1 + tanh) / 2 = s):
 //   g(z) = z s,   g'(z) = s + 2 z s (1 - s) k (1 + 3 a z^2)
@@ -99,15 +111,25 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long x)
 }
 
 // ---------------------------------------------------------------------------
-// subgrid targets of a fine state: one thread per coarse cell
+// subgrid targets of a fine state: one thread per coarse cell.  The filter F
+// is the f x f block mean horizontally and, in z, the coarse-vertex window
+// of closure_zweights: coarse plane K (z = K / (cz - 1)) is the mean of the
+// fine profile's linear interpolant over [z - H/2, z + H/2], H the coarse
+// spacing, and the wall planes are the fine wall planes, so the filtered
+// fields keep the walls' values (T = 1 and 1 + theta, w = 0).
 // ---------------------------------------------------------------------------
 struct TargetArgs {
-    const double* fine[CL_NF];
-    double* coarse[CL_NF];
-    double* target[CL_NOUT];
+    ClSource src;            // the fine fields
+    void* coarse[CL_NF];     // outputs (float when out_f32, else double)
+    void* target[CL_NOUT];
+    int out_f32;
     int nx, ny, f, cx, cy;
     long nc;
     double fmag;
+    const int* zk0;          // per coarse plane: its first fine plane
+    const int* znk;          // and how many fine planes its window covers
+    const double* zw;        // their weights, [cz][zmax]
+    int zmax;
 };
 
 __global__ void __launch_bounds__(256) targets_kernel(TargetArgs a)
@@ -119,29 +141,36 @@ __global__ void __launch_bounds__(256) targets_kernel(TargetArgs a)
     const long rem = q - (long)K * pc;
     const int J = (int)(rem / a.cx), I = (int)(rem - (long)J * a.cx);
     const int f = a.f;
+    const long nxy = (long)a.nx * a.ny;
     double s[CL_NF] = {0, 0, 0, 0, 0, 0, 0, 0};
     double uu[6] = {0, 0, 0, 0, 0, 0}, bb[6] = {0, 0, 0, 0, 0, 0}, uxb[3] = {0, 0, 0}, ut[3] = {0, 0, 0};
-    for (int kk = 0; kk < f; ++kk)
+    const int nk = a.znk[K];
+    for (int kk = 0; kk < nk; ++kk) {
+        const int k = a.zk0[K] + kk;
+        const double wz = a.zw[K * a.zmax + kk];
         for (int jj = 0; jj < f; ++jj)
             for (int ii = 0; ii < f; ++ii) {
-                const long p = ((long)(f * K + kk) * a.ny + (f * J + jj)) * a.nx + (long)(f * I + ii);
+                const long p = (long)k * nxy + (long)(f * J + jj) * a.nx + (long)(f * I + ii);
                 double v[CL_NF];
 #pragma unroll
                 for (int n = 0; n < CL_NF; ++n) {
-                    v[n] = a.fine[n][p];
-                    s[n] += v[n];
+                    double x = a.src.f32[n] ? (double)((const float*)a.src.p[n])[p] : ((const double*)a.src.p[n])[p];
+                    if (a.src.mean[n]) x += a.src.mean[n][k];
+                    v[n] = x + a.src.add[n];
+                    s[n] += wz * v[n];
                 }
                 const double t = v[1], u = v[2], w2 = v[3], w = v[4], bx = v[5], by = v[6], bz = v[7];
-                uu[0] += u * u; uu[1] += w2 * w2; uu[2] += w * w;
-                uu[3] += u * w2; uu[4] += u * w; uu[5] += w2 * w;
-                bb[0] += bx * bx; bb[1] += by * by; bb[2] += bz * bz;
-                bb[3] += bx * by; bb[4] += bx * bz; bb[5] += by * bz;
-                uxb[0] += w2 * bz - w * by;
-                uxb[1] += w * bx - u * bz;
-                uxb[2] += u * by - w2 * bx;
-                ut[0] += u * t; ut[1] += w2 * t; ut[2] += w * t;
+                uu[0] += wz * (u * u); uu[1] += wz * (w2 * w2); uu[2] += wz * (w * w);
+                uu[3] += wz * (u * w2); uu[4] += wz * (u * w); uu[5] += wz * (w2 * w);
+                bb[0] += wz * (bx * bx); bb[1] += wz * (by * by); bb[2] += wz * (bz * bz);
+                bb[3] += wz * (bx * by); bb[4] += wz * (bx * bz); bb[5] += wz * (by * bz);
+                uxb[0] += wz * (w2 * bz - w * by);
+                uxb[1] += wz * (w * bx - u * bz);
+                uxb[2] += wz * (u * by - w2 * bx);
+                ut[0] += wz * (u * t); ut[1] += wz * (w2 * t); ut[2] += wz * (w * t);
             }
-    const double inv = 1.0 / ((double)f * f * f);
+    }
+    const double inv = 1.0 / ((double)f * f);
     double m[CL_NF];
 #pragma unroll
     for (int n = 0; n < CL_NF; ++n) m[n] = s[n] * inv;
@@ -154,15 +183,25 @@ __global__ void __launch_bounds__(256) targets_kernel(TargetArgs a)
         M[c] = bb[c] * inv - B[pi[c]] * B[pj[c]];
     }
     const double trace = M[0] + M[1] + M[2];
+    double o[CL_NOUT];
 #pragma unroll
-    for (int n = 0; n < CL_NF; ++n) a.coarse[n][q] = m[n];
+    for (int c = 0; c < 6; ++c) o[c] = r * R[c] - a.fmag * (M[c] - (c < 3 ? 0.5 * trace : 0.0));
+    o[6] = uxb[0] * inv - (U[1] * B[2] - U[2] * B[1]);
+    o[7] = uxb[1] * inv - (U[2] * B[0] - U[0] * B[2]);
+    o[8] = uxb[2] * inv - (U[0] * B[1] - U[1] * B[0]);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) a.target[c][q] = r * R[c] - a.fmag * (M[c] - (c < 3 ? 0.5 * trace : 0.0));
-    a.target[6][q] = uxb[0] * inv - (U[1] * B[2] - U[2] * B[1]);
-    a.target[7][q] = uxb[1] * inv - (U[2] * B[0] - U[0] * B[2]);
-    a.target[8][q] = uxb[2] * inv - (U[0] * B[1] - U[1] * B[0]);
+    for (int c = 0; c < 3; ++c) o[9 + c] = ut[c] * inv - U[c] * T;
+    if (a.out_f32) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a.target[9 + c][q] = ut[c] * inv - U[c] * T;
+        for (int n = 0; n < CL_NF; ++n) ((float*)a.coarse[n])[q] = (float)m[n];
+#pragma unroll
+        for (int c = 0; c < CL_NOUT; ++c) ((float*)a.target[c])[q] = (float)o[c];
+    } else {
+#pragma unroll
+        for (int n = 0; n < CL_NF; ++n) ((double*)a.coarse[n])[q] = m[n];
+#pragma unroll
+        for (int c = 0; c < CL_NOUT; ++c) ((double*)a.target[c])[q] = o[c];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -187,7 +226,7 @@ __global__ void __launch_bounds__(256) feat_kernel(FeatArgs a)
         double v = a.src.f32[n] ? (double)((const float*)a.src.p[n])[q] : ((const double*)a.src.p[n])[q];
         if (a.src.mean[n]) v += a.src.mean[n][k];
         v += a.src.add[n];
-        g[n] = (__bf16)(float)((v - a.mu[n]) / a.sd[n]);
+        g[n] = feature_bf16((v - a.mu[n]) / a.sd[n]);
     }
     a.grid[q] = g;
 }
@@ -209,7 +248,7 @@ __global__ void __launch_bounds__(256) prep_kernel(PrepArgs a)
     const float* r = a.raw + snap * 20 * a.n3 + q;
     bf16x8 v;
 #pragma unroll
-    for (int n = 0; n < CL_NF; ++n) v[n] = (__bf16)(float)(((double)r[n * a.n3] - a.mu[n]) / a.sd[n]);
+    for (int n = 0; n < CL_NF; ++n) v[n] = feature_bf16(((double)r[n * a.n3] - a.mu[n]) / a.sd[n]);
     a.grid[g] = v;
 #pragma unroll
     for (int o = 0; o < CL_NOUT; ++o)
@@ -834,6 +873,92 @@ namespace {
 
 bool ok(hipError_t e) { return e == hipSuccess; }
 
+// z-weights of the coarse-vertex filter: fine plane k at z_k = k / (nz - 1),
+// coarse plane K at Z_K = K / (cz - 1).  The wall planes take the fine wall
+// plane; an interior plane takes the mean over [Z_K - H/2, Z_K + H/2]
+// (H = 1 / (cz - 1)) of the fine profile's linear interpolant, integrated
+// exactly interval by interval, the weights normalised to sum to 1.
+// Rows: k0[K], nk[K], weights w[K * zmax + i] of planes k0[K] + i.
+void closure_zweights(int nz, int cz, std::vector<int>& k0, std::vector<int>& nk, std::vector<double>& w,
+                      int& zmax)
+{
+    std::vector<std::vector<double>> rows(cz);
+    k0.assign(cz, 0);
+    for (int K = 0; K < cz; ++K) {
+        if (K == 0 || K == cz - 1) {
+            k0[K] = K == 0 ? 0 : nz - 1;
+            rows[K] = {1.0};
+            continue;
+        }
+        const double Z = (double)K / (double)(cz - 1), half = 0.5 / (double)(cz - 1);
+        const double a = Z - half, b = Z + half;
+        int kl = (int)std::floor(a * (double)(nz - 1)), kh = (int)std::floor(b * (double)(nz - 1));
+        kl = kl < 0 ? 0 : kl;
+        kh = kh > nz - 2 ? nz - 2 : kh;
+        k0[K] = kl;
+        std::vector<double> r(kh - kl + 2, 0.0);
+        for (int k = kl; k <= kh; ++k) {
+            const double zk = (double)k / (double)(nz - 1), zk1 = (double)(k + 1) / (double)(nz - 1);
+            const double p = a > zk ? a : zk, q = b < zk1 ? b : zk1;
+            if (!(q > p)) continue;
+            const double dk = zk1 - zk;
+            const double tp = (p - zk) / dk, tq = (q - zk) / dk;
+            r[k - kl] += dk * ((tq - tp) - 0.5 * (tq * tq - tp * tp));
+            r[k - kl + 1] += dk * (0.5 * (tq * tq - tp * tp));
+        }
+        double sum = 0.0;
+        for (double x : r) sum += x;
+        for (double& x : r) x /= sum;
+        rows[K] = r;
+    }
+    zmax = 1;
+    for (const auto& r : rows) zmax = (int)r.size() > zmax ? (int)r.size() : zmax;
+    nk.assign(cz, 0);
+    w.assign((size_t)cz * zmax, 0.0);
+    for (int K = 0; K < cz; ++K) {
+        nk[K] = (int)rows[K].size();
+        for (int i = 0; i < nk[K]; ++i) w[(size_t)K * zmax + i] = rows[K][i];
+    }
+}
+
+// the fine fields of src filtered into coarse fields and targets (float when
+// out_f32) on stream s; the z-weight tables go to `zmem` (device memory of
+// closure_ztable_bytes), copied from the host synchronously
+size_t closure_ztable_bytes(int cz, int zmax) { return (size_t)cz * (2 * sizeof(int) + zmax * sizeof(double)); }
+
+int launch_targets(const ClSource& src, int nx, int ny, int nz, int f, double fmag, void* const coarse[CL_NF],
+                   void* const target[CL_NOUT], int out_f32, void** zmem, hipStream_t s)
+{
+    const int cx = nx / f, cy = ny / f, cz = nz / f;
+    if (cz < 2) return MHD_E_ARG;
+    std::vector<int> k0, nk;
+    std::vector<double> w;
+    int zmax = 1;
+    closure_zweights(nz, cz, k0, nk, w, zmax);
+    const size_t zb = closure_ztable_bytes(cz, zmax);
+    if (!ok(hipMalloc(zmem, zb))) return MHD_E_ALLOC;
+    std::vector<unsigned char> h(zb);
+    memcpy(h.data(), w.data(), w.size() * sizeof(double));
+    memcpy(h.data() + w.size() * sizeof(double), k0.data(), cz * sizeof(int));
+    memcpy(h.data() + w.size() * sizeof(double) + cz * sizeof(int), nk.data(), cz * sizeof(int));
+    if (!ok(hipMemcpy(*zmem, h.data(), zb, hipMemcpyHostToDevice))) return MHD_E_DEVICE;
+    TargetArgs a;
+    memset(&a, 0, sizeof(a));
+    a.src = src;
+    for (int n = 0; n < CL_NF; ++n) a.coarse[n] = coarse[n];
+    for (int o = 0; o < CL_NOUT; ++o) a.target[o] = target[o];
+    a.out_f32 = out_f32;
+    a.nx = nx; a.ny = ny; a.f = f; a.cx = cx; a.cy = cy;
+    a.nc = (long)cx * cy * cz;
+    a.fmag = fmag;
+    a.zw = (const double*)*zmem;
+    a.zk0 = (const int*)((const unsigned char*)*zmem + w.size() * sizeof(double));
+    a.znk = a.zk0 + cz;
+    a.zmax = zmax;
+    hipLaunchKernelGGL(targets_kernel, dim3(nblk(a.nc)), dim3(256), 0, s, a);
+    return ok(hipGetLastError()) ? 0 : MHD_E_DEVICE;
+}
+
 int ensure(void** p, long* cap, long want, size_t elem)
 {
     if (*cap >= want) return 0;
@@ -937,17 +1062,138 @@ int run_train(mhd_closure* c, int batch, const long* dcells, long step)
     return ok(hipGetLastError()) ? 0 : MHD_E_DEVICE;
 }
 
+// room for one more snapshot of n3 cells in the data set (the store doubles
+// its capacity, keeping its contents)
+int grow_store(mhd_closure* c, long n3)
+{
+    if (c->nsnap == c->cap) {
+        // grow the snapshot store (double the capacity, keep the contents)
+        const int ncap = c->cap ? 2 * c->cap : 4;
+        float *raw = nullptr, *tn = nullptr;
+        bf16x8* grid = nullptr;
+        if (!ok(hipMalloc((void**)&raw, (size_t)ncap * 20 * n3 * 4)) ||
+            !ok(hipMalloc((void**)&tn, (size_t)ncap * 12 * n3 * 4)) ||
+            !ok(hipMalloc((void**)&grid, (size_t)ncap * n3 * sizeof(bf16x8)))) {
+            for (void* p : {(void*)raw, (void*)tn, (void*)grid})
+                if (p) (void)hipFree(p);
+            return MHD_E_ALLOC;
+        }
+        if ((c->nsnap &&
+             !ok(hipMemcpyAsync(raw, c->raw, (size_t)c->nsnap * 20 * n3 * 4, hipMemcpyDeviceToDevice, c->s))) ||
+            !ok(hipStreamSynchronize(c->s))) {
+            for (void* p : {(void*)raw, (void*)tn, (void*)grid}) (void)hipFree(p);
+            return MHD_E_DEVICE;
+        }
+        for (void* p : {(void*)c->raw, (void*)c->tn, (void*)c->grid})
+            if (p) (void)hipFree(p);
+        c->raw = raw;
+        c->tn = tn;
+        c->grid = grid;
+        c->cap = ncap;
+    }
+    return 0;
+}
+
+// per-block (sum, sum of squares) of the 20 arrays of one snapshot
+__global__ void __launch_bounds__(256) snap_stats_kernel(const float* __restrict__ raw, long n3, int nblk_per,
+                                                         double* __restrict__ part)
+{
+    const int arr = blockIdx.x / nblk_per, b = blockIdx.x - arr * nblk_per;
+    const long lo = n3 * b / nblk_per, hi = n3 * (b + 1) / nblk_per;
+    const float* x = raw + (long)arr * n3;
+    double s = 0.0, s2 = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += 256) {
+        const double v = x[i];
+        s += v;
+        s2 += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+    }
+    __shared__ double sh[2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][wave] = s;
+        sh[1][wave] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
+        part[2 * blockIdx.x + 1] = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+    }
+}
+
+}  // namespace
+
+namespace mhd {
+
+int closure_add_source(mhd_closure* c, const ClSource& src, int nx, int ny, int nz, int f, double fmag)
+{
+    if (f < 1 || nx % f || ny % f || nz % f || nz / f < 2) return MHD_E_ARG;
+    const int cx = nx / f, cy = ny / f, cz = nz / f;
+    if (c->nsnap && (cx != c->cx || cy != c->cy || cz != c->cz)) return MHD_E_ARG;
+    const long n3 = (long)cx * cy * cz;
+    {
+        const int e = grow_store(c, n3);
+        if (e) return e;
+    }
+    float* snap = c->raw + (size_t)c->nsnap * 20 * n3;
+    void* co[CL_NF];
+    void* tg[CL_NOUT];
+    for (int n = 0; n < CL_NF; ++n) co[n] = snap + (size_t)n * n3;
+    for (int o = 0; o < CL_NOUT; ++o) tg[o] = snap + (size_t)(CL_NF + o) * n3;
+    void* zmem = nullptr;
+    double* part = nullptr;
+    constexpr int NBLK = 64;
+    int rc = launch_targets(src, nx, ny, nz, f, fmag, co, tg, 1, &zmem, c->s);
+    if (!rc && !ok(hipMalloc((void**)&part, 20 * NBLK * 2 * sizeof(double)))) rc = MHD_E_ALLOC;
+    std::vector<double> hp(20 * NBLK * 2);
+    if (!rc) {
+        hipLaunchKernelGGL(snap_stats_kernel, dim3(20 * NBLK), dim3(256), 0, c->s, (const float*)snap, n3, NBLK,
+                           part);
+        if (!ok(hipGetLastError()) ||
+            !ok(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c->s)) ||
+            !ok(hipStreamSynchronize(c->s)))
+            rc = MHD_E_DEVICE;
+    }
+    if (zmem) (void)hipFree(zmem);
+    if (part) (void)hipFree(part);
+    if (rc) return rc;
+    // the snapshot is in place: the shape and statistics change now
+    c->cx = cx; c->cy = cy; c->cz = cz; c->n3 = n3;
+    for (int n = 0; n < 20; ++n) {
+        double sm = 0.0, sm2 = 0.0;
+        for (int b = 0; b < NBLK; ++b) {
+            sm += hp[(n * NBLK + b) * 2];
+            sm2 += hp[(n * NBLK + b) * 2 + 1];
+        }
+        c->sum[n] += sm;
+        c->sum2[n] += sm2;
+    }
+    c->count += n3;
+    c->nsnap += 1;
+    c->dirty = 1;
+    return 0;
+}
+
+}  // namespace mhd
+
+namespace {
 }  // namespace
 
 namespace mhd {
 
 int closure_fluxes(mhd_closure* cl, const ClSource& src, int nx, int ny, int nz, void* const out[CL_NOUT],
-                   int out_f32, hipStream_t s)
+                   int out_f32, void* grid, hipStream_t s)
 {
     const long n = (long)nx * ny * nz;
-    {
+    static_assert(sizeof(bf16x8) == 16, "feature grid element");
+    bf16x8* fg = (bf16x8*)grid;
+    if (!fg) {
         int e = ensure((void**)&cl->sgrid, &cl->sgrid_n, n, sizeof(bf16x8));
         if (e) return e;
+        fg = cl->sgrid;
     }
     FeatArgs f;
     memset(&f, 0, sizeof(f));
@@ -955,10 +1201,10 @@ int closure_fluxes(mhd_closure* cl, const ClSource& src, int nx, int ny, int nz,
     f.nxy = nx * ny;
     f.n = n;
     for (int q = 0; q < CL_NF; ++q) { f.mu[q] = cl->norm[q]; f.sd[q] = cl->norm[8 + q]; }
-    f.grid = cl->sgrid;
+    f.grid = fg;
     hipLaunchKernelGGL(feat_kernel, dim3(nblk(n)), dim3(256), 0, s, f);
     FwdArgs a = fwd_args(cl);
-    a.grid = cl->sgrid;
+    a.grid = fg;
     a.nx = nx; a.ny = ny; a.nz = nz;
     a.n3 = n;
     a.n = n;
@@ -1084,31 +1330,35 @@ int mhd_closure_targets(int nx, int ny, int nz, int f, double fmag, const double
         if (!target[o]) return MHD_E_ARG;
     const long n3 = (long)nx * ny * nz;
     const int cx = nx / f, cy = ny / f, cz = nz / f;
+    if (cz < 2) return MHD_E_ARG;
     const long nc = (long)cx * cy * cz;
     double* d = nullptr;
+    void* zmem = nullptr;
     const size_t bytes = (size_t)(8 * n3 + 20 * nc) * sizeof(double);
     if (!ok(hipMalloc((void**)&d, bytes))) return MHD_E_ALLOC;
     int rc = MHD_E_DEVICE;
-    TargetArgs a;
-    memset(&a, 0, sizeof(a));
     do {
+        ClSource src;
+        memset(&src, 0, sizeof(src));
         bool up = true;
         for (int n = 0; n < 8 && up; ++n) {
-            a.fine[n] = d + n * n3;
-            up = ok(hipMemcpy((void*)a.fine[n], fine[n], (size_t)n3 * 8, hipMemcpyHostToDevice));
+            src.p[n] = d + n * n3;
+            up = ok(hipMemcpy((void*)src.p[n], fine[n], (size_t)n3 * 8, hipMemcpyHostToDevice));
         }
         if (!up) break;
-        for (int n = 0; n < 8; ++n) a.coarse[n] = d + 8 * n3 + n * nc;
-        for (int o = 0; o < 12; ++o) a.target[o] = d + 8 * n3 + (8 + o) * nc;
-        a.nx = nx; a.ny = ny; a.f = f; a.cx = cx; a.cy = cy; a.nc = nc; a.fmag = fmag;
-        hipLaunchKernelGGL(targets_kernel, dim3(nblk(nc)), dim3(256), 0, 0, a);
-        if (!ok(hipGetLastError())) break;
+        void* co[8];
+        void* tg[12];
+        for (int n = 0; n < 8; ++n) co[n] = d + 8 * n3 + n * nc;
+        for (int o = 0; o < 12; ++o) tg[o] = d + 8 * n3 + (8 + o) * nc;
+        const int e = launch_targets(src, nx, ny, nz, f, fmag, co, tg, 0, &zmem, nullptr);
+        if (e) { rc = e; break; }
         bool dn = true;
-        for (int n = 0; n < 8 && dn; ++n) dn = ok(hipMemcpy(coarse[n], a.coarse[n], (size_t)nc * 8, hipMemcpyDeviceToHost));
-        for (int o = 0; o < 12 && dn; ++o) dn = ok(hipMemcpy(target[o], a.target[o], (size_t)nc * 8, hipMemcpyDeviceToHost));
+        for (int n = 0; n < 8 && dn; ++n) dn = ok(hipMemcpy(coarse[n], co[n], (size_t)nc * 8, hipMemcpyDeviceToHost));
+        for (int o = 0; o < 12 && dn; ++o) dn = ok(hipMemcpy(target[o], tg[o], (size_t)nc * 8, hipMemcpyDeviceToHost));
         if (dn) rc = MHD_OK;
     } while (0);
     (void)hipFree(d);
+    if (zmem) (void)hipFree(zmem);
     return rc;
 }
 
@@ -1117,49 +1367,39 @@ int mhd_closure_add(mhd_closure_t* c, int cx, int cy, int cz, const double* cons
 {
     if (!c || !coarse || !target || cx < 1 || cy < 1 || cz < 1) return MHD_E_ARG;
     if (c->nsnap && (cx != c->cx || cy != c->cy || cz != c->cz)) return MHD_E_ARG;
+    // every input is checked before any state changes: a rejected call leaves
+    // the data set, its statistics and its shape as they were
+    for (int n = 0; n < 8; ++n)
+        if (!coarse[n]) return MHD_E_ARG;
+    for (int o = 0; o < 12; ++o)
+        if (!target[o]) return MHD_E_ARG;
     const long n3 = (long)cx * cy * cz;
-    if (c->nsnap == c->cap) {
-        // grow the snapshot store (double the capacity, keep the contents)
-        const int ncap = c->cap ? 2 * c->cap : 4;
-        float *raw = nullptr, *tn = nullptr;
-        bf16x8* grid = nullptr;
-        if (!ok(hipMalloc((void**)&raw, (size_t)ncap * 20 * n3 * 4)) ||
-            !ok(hipMalloc((void**)&tn, (size_t)ncap * 12 * n3 * 4)) ||
-            !ok(hipMalloc((void**)&grid, (size_t)ncap * n3 * sizeof(bf16x8)))) {
-            for (void* p : {(void*)raw, (void*)tn, (void*)grid})
-                if (p) (void)hipFree(p);
-            return MHD_E_ALLOC;
-        }
-        if (c->nsnap &&
-            !ok(hipMemcpyAsync(raw, c->raw, (size_t)c->nsnap * 20 * n3 * 4, hipMemcpyDeviceToDevice, c->s)))
-            return MHD_E_DEVICE;
-        (void)hipStreamSynchronize(c->s);
-        for (void* p : {(void*)c->raw, (void*)c->tn, (void*)c->grid})
-            if (p) (void)hipFree(p);
-        c->raw = raw;
-        c->tn = tn;
-        c->grid = grid;
-        c->cap = ncap;
+    {
+        const int e = grow_store(c, n3);
+        if (e) return e;
     }
-    c->cx = cx; c->cy = cy; c->cz = cz; c->n3 = n3;
     std::vector<float> h((size_t)20 * n3);
+    double s[20], s2[20];
     for (int n = 0; n < 20; ++n) {
         const double* src = n < 8 ? coarse[n] : target[n - 8];
-        if (!src) return MHD_E_ARG;
-        double s = 0.0, s2 = 0.0;
+        s[n] = s2[n] = 0.0;
         for (long q = 0; q < n3; ++q) {
             const float v = (float)src[q];
             h[(size_t)n * n3 + q] = v;
-            s += v;
-            s2 += (double)v * v;
+            s[n] += v;
+            s2[n] += (double)v * v;
         }
-        c->sum[n] += s;
-        c->sum2[n] += s2;
     }
-    c->count += n3;
     if (!ok(hipMemcpyAsync(c->raw + (size_t)c->nsnap * 20 * n3, h.data(), h.size() * 4, hipMemcpyHostToDevice, c->s)))
         return MHD_E_DEVICE;
     if (!ok(hipStreamSynchronize(c->s))) return MHD_E_DEVICE;
+    // the snapshot is in place: only now do the shape and statistics change
+    c->cx = cx; c->cy = cy; c->cz = cz; c->n3 = n3;
+    for (int n = 0; n < 20; ++n) {
+        c->sum[n] += s[n];
+        c->sum2[n] += s2[n];
+    }
+    c->count += n3;
     c->nsnap += 1;
     c->dirty = 1;
     return 0;
@@ -1290,7 +1530,7 @@ int mhd_closure_predict(mhd_closure_t* c, int nx, int ny, int nz, const double* 
         if (!up) break;
         void* o[CL_NOUT];
         for (int q = 0; q < CL_NOUT; ++q) o[q] = d + (8 + q) * n;
-        if (closure_fluxes(c, src, nx, ny, nz, o, 0, c->s)) break;
+        if (closure_fluxes(c, src, nx, ny, nz, o, 0, nullptr, c->s)) break;
         bool dn = true;
         for (int q = 0; q < CL_NOUT && dn; ++q) {
             if (!out[q]) { rc = MHD_E_ARG; dn = false; break; }

@@ -21,10 +21,21 @@ struct ClSource {
     double add[CL_NF];
 };
 
+// bytes of the feature grid of a box of n cells (closure_fluxes' `grid`)
+constexpr size_t closure_grid_bytes(long n) { return (size_t)n * 16; }
+
 // features of the solver's state and the network's 12 fluxes (de-normalised)
-// into out[12] (float or double arrays of the box's size) on stream s
+// into out[12] (float or double arrays of the box's size) on stream s.  The
+// features go to `grid` (closure_grid_bytes(n) of device memory owned by the
+// caller, so runs sharing one closure never share it), or, when grid is
+// null, to the closure's own buffer (calls on the closure's stream only).
 int closure_fluxes(mhd_closure* cl, const ClSource& src, int nx, int ny, int nz, void* const out[CL_NOUT],
-                   int out_f32, hipStream_t s);
+                   int out_f32, void* grid, hipStream_t s);
+
+// appends to the closure's data set the coarse fields and subgrid targets of
+// a fine state given as sources on the device (the fine run's own arrays):
+// nothing crosses to the host but 20 pairs of sums for the normalisation
+int closure_add_source(mhd_closure* cl, const ClSource& src, int nx, int ny, int nz, int f, double fmag);
 
 // momentum and T right-hand sides of the closure (newest history level):
 // dots[m] -= ((dx[m] + dy[m]) + dz[m]) / r for u v w (m = 0 1 2), and

@@ -23,6 +23,7 @@
 // d2bydx / d2bydy at the Nyquist mode.
 #include <type_traits>
 
+#include "launch.hpp"
 #include "physics.hpp"
 #include "zstencil.hpp"
 
@@ -58,13 +59,20 @@ __device__ __forceinline__ double dupi(const T* f, int c, double h, bool up)
 // WALL = false: interior planes [4, nz-4), zc per thread, register windows;
 // WALL = true: one boundary row per thread (planes 0..3 and nz-4..nz-1).
 template <typename TW, bool MIX, bool WALL>
-__global__ void __launch_bounds__(HNT) hydro_col_kernel(HydroColArgs av)
+struct HydroColKernel {
+    using Args = HydroColArgs;
+    static constexpr int NT = HNT;
+    static __device__ __forceinline__ void body(const HydroColArgs& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TW, bool MIX, bool WALL>
+__device__ __forceinline__ void HydroColKernel<TW, MIX, WALL>::body(const HydroColArgs& a, uint3 bid, uint3 gdim)
 {
-    (void)av;
-    const HydroColArgs& a = kernarg<HydroColArgs>();
+    (void)bid;
+    (void)gdim;
     using TS = typename std::conditional<MIX, float, double>::type;
     const int nxy = a.g.nxy, nz = a.g.nz;
-    const long col = (long)blockIdx.x * HNT + threadIdx.x;
+    const long col = (long)bid.x * HNT + threadIdx.x;
     if (col >= nxy) return;
     const double h = a.p.ho12, s = a.p.sc1;
     const double F = a.p.f, fo2 = a.p.fo2, scko3 = a.p.scko3;
@@ -277,12 +285,12 @@ __global__ void __launch_bounds__(HNT) hydro_col_kernel(HydroColArgs av)
 
     if constexpr (WALL) {
         const int nb = nz < 8 ? nz : 8;
-        const int pr = (int)blockIdx.y;
+        const int pr = (int)bid.y;
         slow(pr < 4 ? pr : nz - (nb - pr));
         return;
     }
     // planes [4, nz-4) use the interior stencils of every derivative
-    const int kf0 = 4 + (int)blockIdx.y * a.zc;
+    const int kf0 = 4 + (int)bid.y * a.zc;
     const int kf1 = (kf0 + a.zc < nz - 4) ? kf0 + a.zc : nz - 4;
     if (kf0 < kf1) {
         // windows: centre index 3 of the 7-wide, 4 of the 9-wide (w), 2 of the 5-wide
@@ -373,13 +381,13 @@ int launch_hydro_col(const HydroColArgs& a, int mixed, hipStream_t s)
     const int nin = a.g.nz - 8;                   // interior planes
     if (nin > 0) {
         const dim3 grid(bx, (unsigned)((nin + a.zc - 1) / a.zc));
-        if (mixed) hipLaunchKernelGGL((hydro_col_kernel<float, true, false>), grid, dim3(HNT), 0, s, a);
-        else hipLaunchKernelGGL((hydro_col_kernel<double, false, false>), grid, dim3(HNT), 0, s, a);
+        const int e = mixed ? mhd_launch<HydroColKernel<float, true, false>>(grid, dim3(HNT), 0, s, a)
+                            : mhd_launch<HydroColKernel<double, false, false>>(grid, dim3(HNT), 0, s, a);
+        if (e) return e;
     }
     const dim3 gw(bx, (unsigned)(a.g.nz < 8 ? a.g.nz : 8));
-    if (mixed) hipLaunchKernelGGL((hydro_col_kernel<float, true, true>), gw, dim3(HNT), 0, s, a);
-    else hipLaunchKernelGGL((hydro_col_kernel<double, false, true>), gw, dim3(HNT), 0, s, a);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
+    return mixed ? mhd_launch<HydroColKernel<float, true, true>>(gw, dim3(HNT), 0, s, a)
+                 : mhd_launch<HydroColKernel<double, false, true>>(gw, dim3(HNT), 0, s, a);
 }
 
 }  // namespace mhd

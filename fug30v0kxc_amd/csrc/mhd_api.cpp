@@ -21,11 +21,20 @@
 #include <vector>
 
 #include "closure.hpp"
+#include "launch.hpp"
 #include "physics.hpp"
 #include "spectral.hpp"
 #include "zstencil.hpp"
 
 using namespace mhd;
+
+namespace mhd {
+LaunchRecorder*& launch_recorder()
+{
+    static thread_local LaunchRecorder* r = nullptr;
+    return r;
+}
+}  // namespace mhd
 
 int mhd_hip_error(hipError_t e, const char* what, const char* file, int line)
 {
@@ -81,7 +90,7 @@ struct mhd_ctx {
     // through the load's extra term)
     mhd_closure* closure;
     void* cl_mem;
-    void *cflux[12], *cdx[4], *cdy[4], *cdz[4];
+    void *cflux[12], *cdx[4], *cdy[4], *cdz[4], *cgrid;
     double* cneg1;
     int eta_on;                      // eta(z) extension active
     double *etar, *eta1;             // eta(z)/eta_0 and zeta C_k (etar - 1), nz entries
@@ -526,7 +535,7 @@ int run_closure_flux(mhd_ctx* c)
         src.mean[6] = c->bybar;
         src.add[7] = c->sc.bzbar;
     }
-    const int e = closure_fluxes(c->closure, src, c->nx, c->ny, c->nz, c->cflux, c->wf32, c->s);
+    const int e = closure_fluxes(c->closure, src, c->nx, c->ny, c->nz, c->cflux, c->wf32, c->cgrid, c->s);
     return e ? (e == MHD_E_ALLOC ? e : MHD_E_DEVICE) : 0;
 }
 
@@ -1504,8 +1513,10 @@ int mhd_step(mhd_ctx_t* c, int nsteps, double dt_fixed, int ndu06, mhd_diag_t* l
 // ENS_BLOCK steps.  With a fixed dt the AB3 coefficients are constant after
 // the Euler and AB2 start-up steps, the history ring has period 3 and the
 // dealias cadence period 10, so the kernel sequence of 30 steps repeats
-// exactly.  It is captured once, every box on its own stream branch
-// (fork/join through events), and replayed.
+// exactly.  Each box's sequence is recorded (launch.hpp); the k-th launches
+// of all boxes of a class (same kernels, i.e. the same lengths, precision
+// and options) become one launch over the table of their argument blocks,
+// and the graph holds those batched launches, one branch per class.
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
@@ -1514,10 +1525,13 @@ struct mhd_ensemble {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
+    std::vector<hipStream_t> branch;   // one stream per class of boxes
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    void* tables = nullptr;            // device: argument blocks, starts and grids of the batched launches
     double dt = 0.0;
     long nodes = 0;
+    int classes = 0;
 };
 
 namespace {
@@ -1535,9 +1549,158 @@ void ens_free(mhd_ensemble* e)
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (hipEvent_t ev : e->join)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t st : e->branch)
+        if (st) (void)hipStreamDestroy(st);
     if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->s) (void)hipStreamDestroy(e->s);
+    if (e->tables) (void)hipFree(e->tables);
     delete e;
+}
+
+// The kernel launches of ENS_BLOCK steps of one box, recorded instead of
+// issued.  The box's stream is put in capture mode meanwhile: any launch that
+// bypassed the recorder would land in that throwaway graph, and is reported
+// as MHD_E_STATE.  The host-side bookkeeping (driver scalars, history ring)
+// is put back afterwards; the state ping-pong swaps an even number of times.
+int record_block(mhd_ctx* c, double dt, LaunchRecorder& rec)
+{
+    const mhd_scalars_t keep = c->sc;
+    const int ring[3] = {c->ring[0], c->ring[1], c->ring[2]};
+    MHD_HIP_CHECK(hipStreamBeginCapture(c->s, hipStreamCaptureModeRelaxed));
+    launch_recorder() = &rec;
+    const int rc = mhd_step(c, ENS_BLOCK, dt, 0, nullptr);
+    launch_recorder() = nullptr;
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(c->s, &g);
+    size_t stray = 0;
+    if (g) {
+        (void)hipGraphGetNodes(g, nullptr, &stray);
+        (void)hipGraphDestroy(g);
+    }
+    c->sc = keep;
+    for (int l = 0; l < 3; ++l) c->ring[l] = ring[l];
+    if (rc) return rc;
+    if (ce != hipSuccess) return MHD_E_DEVICE;
+    return stray ? MHD_E_STATE : 0;
+}
+
+// Boxes whose recorded sequences launch the same kernels with the same
+// workgroup sizes (same lengths, precision and options) form a class; the
+// k-th launches of a class's boxes become one batched launch over a table of
+// the boxes' argument blocks and grids (launch.hpp).  Classes run on
+// parallel branches of the graph.
+int build_batched(mhd_ensemble* e)
+{
+    const int nbox = (int)e->box.size();
+    std::vector<LaunchRecorder> rec(nbox);
+    for (int b = 0; b < nbox; ++b) TRY(record_block(e->box[b], e->dt, rec[b]));
+    // classes by signature
+    std::vector<std::vector<int>> cls;
+    auto same = [&](int a, int b) {
+        const auto& x = rec[a].recs;
+        const auto& y = rec[b].recs;
+        if (x.size() != y.size()) return false;
+        for (size_t k = 0; k < x.size(); ++k)
+            if (x[k].kernel != y[k].kernel || x[k].block.x != y[k].block.x || x[k].args.size() != y[k].args.size())
+                return false;
+        return true;
+    };
+    for (int b = 0; b < nbox; ++b) {
+        bool put = false;
+        for (auto& c : cls)
+            if (same(c[0], b)) {
+                c.push_back(b);
+                put = true;
+                break;
+            }
+        if (!put) cls.push_back({b});
+    }
+    for (int b = 0; b < nbox; ++b)
+        for (const LaunchRec& r : rec[b].recs)
+            if (r.block.y != 1 || r.block.z != 1) return MHD_E_STATE;   // 1-D workgroups only
+    // tables: per batched launch, the argument blocks, nbox + 1 starts and nbox grids
+    struct Batch {
+        size_t args_off, start_off, grid_off;
+        unsigned total;
+        size_t lds;
+        int cls, k;
+    };
+    std::vector<Batch> batches;
+    std::vector<unsigned char> host;
+    auto put = [&](const void* p, size_t n) {
+        size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + n);
+        memcpy(host.data() + off, p, n);
+        return off;
+    };
+    for (int ci = 0; ci < (int)cls.size(); ++ci) {
+        const auto& members = cls[ci];
+        const size_t nk = rec[members[0]].recs.size();
+        for (size_t k = 0; k < nk; ++k) {
+            const size_t stride = rec[members[0]].recs[k].args.size();
+            std::vector<unsigned char> blob(stride * members.size());
+            std::vector<unsigned> start(members.size() + 1, 0u);
+            std::vector<uint3> grid(members.size());
+            size_t lds = 0;
+            for (size_t m = 0; m < members.size(); ++m) {
+                const LaunchRec& r = rec[members[m]].recs[k];
+                memcpy(blob.data() + m * stride, r.args.data(), stride);
+                grid[m] = uint3{r.grid.x, r.grid.y, r.grid.z};
+                const unsigned long long nb = (unsigned long long)r.grid.x * r.grid.y * r.grid.z;
+                if (start[m] + nb > 0xffffffffULL) return MHD_E_ARG;
+                start[m + 1] = start[m] + (unsigned)nb;
+                lds = r.lds > lds ? r.lds : lds;
+            }
+            Batch bt;
+            bt.args_off = put(blob.data(), blob.size());
+            bt.start_off = put(start.data(), start.size() * sizeof(unsigned));
+            bt.grid_off = put(grid.data(), grid.size() * sizeof(uint3));
+            bt.total = start.back();
+            bt.lds = lds;
+            bt.cls = ci;
+            bt.k = (int)k;
+            batches.push_back(bt);
+        }
+    }
+    MHD_HIP_CHECK(hipMalloc(&e->tables, host.size() ? host.size() : 256));
+    MHD_HIP_CHECK(hipMemcpy(e->tables, host.data(), host.size(), hipMemcpyHostToDevice));
+    e->classes = (int)cls.size();
+    e->branch.assign(cls.size(), nullptr);
+    e->join.assign(cls.size(), nullptr);
+    for (size_t ci = 0; ci < cls.size(); ++ci) {
+        MHD_HIP_CHECK(hipStreamCreateWithFlags(&e->branch[ci], hipStreamNonBlocking));
+        MHD_HIP_CHECK(hipEventCreateWithFlags(&e->join[ci], hipEventDisableTiming));
+    }
+    // capture: one branch per class, its batched launches in recorded order
+    MHD_HIP_CHECK(hipStreamBeginCapture(e->s, hipStreamCaptureModeRelaxed));
+    int rc = 0;
+    if (hipEventRecord(e->fork, e->s) != hipSuccess) rc = MHD_E_DEVICE;
+    for (size_t ci = 0; ci < cls.size() && !rc; ++ci)
+        if (hipStreamWaitEvent(e->branch[ci], e->fork, 0) != hipSuccess) rc = MHD_E_DEVICE;
+    const unsigned char* dev = (const unsigned char*)e->tables;
+    for (const Batch& bt : batches) {
+        if (rc) break;
+        const auto& members = cls[bt.cls];
+        const LaunchRec& r0 = rec[members[0]].recs[bt.k];
+        BatchTable t;
+        t.args = dev + bt.args_off;
+        t.start = (const unsigned*)(dev + bt.start_off);
+        t.grid = (const uint3*)(dev + bt.grid_off);
+        t.nbox = (int)members.size();
+        t.stride = (int)r0.args.size();
+        if (bt.total > 0 && r0.batch(t, bt.total, r0.block.x, bt.lds, e->branch[bt.cls])) rc = MHD_E_DEVICE;
+    }
+    for (size_t ci = 0; ci < cls.size() && !rc; ++ci) {
+        if (hipEventRecord(e->join[ci], e->branch[ci]) != hipSuccess ||
+            hipStreamWaitEvent(e->s, e->join[ci], 0) != hipSuccess)
+            rc = MHD_E_DEVICE;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->s, &g);
+    e->graph = g;
+    if (rc) return rc;
+    if (ce != hipSuccess || !g) return MHD_E_DEVICE;
+    return 0;
 }
 }  // namespace
 
@@ -1565,7 +1728,6 @@ int mhd_ensemble_create(mhd_ctx_t* const* boxes, int nbox, double dt, mhd_ensemb
     mhd_ensemble* e = new mhd_ensemble();
     e->box.assign(boxes, boxes + nbox);
     e->dt = dt;
-    e->join.assign(nbox, nullptr);
     auto fail = [&](int code) {
         ens_free(e);
         return code;
@@ -1573,36 +1735,11 @@ int mhd_ensemble_create(mhd_ctx_t* const* boxes, int nbox, double dt, mhd_ensemb
     if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
         return fail(MHD_E_DEVICE);
-    for (int b = 0; b < nbox; ++b)
-        if (hipEventCreateWithFlags(&e->join[b], hipEventDisableTiming) != hipSuccess) return fail(MHD_E_DEVICE);
-
-    // capture: the host-side bookkeeping of the steps runs on copies that are
-    // put back afterwards (nothing executes during capture)
-    std::vector<mhd_scalars_t> keep(nbox);
-    std::vector<std::array<int, 3>> ring(nbox);
-    for (int b = 0; b < nbox; ++b) {
-        keep[b] = e->box[b]->sc;
-        ring[b] = {e->box[b]->ring[0], e->box[b]->ring[1], e->box[b]->ring[2]};
+    {
+        const int rc = build_batched(e);
+        if (rc) return fail(rc);
     }
-    if (hipStreamBeginCapture(e->s, hipStreamCaptureModeRelaxed) != hipSuccess) return fail(MHD_E_DEVICE);
-    int rc = 0;
-    if (hipEventRecord(e->fork, e->s) != hipSuccess) rc = MHD_E_DEVICE;
-    for (int b = 0; b < nbox && !rc; ++b) {
-        mhd_ctx* c = e->box[b];
-        if (hipStreamWaitEvent(c->s, e->fork, 0) != hipSuccess) { rc = MHD_E_DEVICE; break; }
-        rc = mhd_step(c, ENS_BLOCK, dt, 0, nullptr);
-        if (!rc && hipEventRecord(e->join[b], c->s) != hipSuccess) rc = MHD_E_DEVICE;
-        if (!rc && hipStreamWaitEvent(e->s, e->join[b], 0) != hipSuccess) rc = MHD_E_DEVICE;
-    }
-    hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(e->s, &g);
-    e->graph = g;
-    for (int b = 0; b < nbox; ++b) {
-        e->box[b]->sc = keep[b];
-        for (int l = 0; l < 3; ++l) e->box[b]->ring[l] = ring[b][l];
-    }
-    if (rc) return fail(rc);
-    if (ce != hipSuccess || !g) return fail(MHD_E_DEVICE);
+    hipGraph_t g = e->graph;
     size_t nn = 0;
     if (hipGraphGetNodes(g, nullptr, &nn) == hipSuccess) e->nodes = (long)nn;
     if (hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess) return fail(MHD_E_DEVICE);
@@ -1740,6 +1877,7 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
 }
 
 long mhd_ensemble_nodes(mhd_ensemble_t* e) { return e ? e->nodes : 0; }
+int mhd_ensemble_classes(mhd_ensemble_t* e) { return e ? e->classes : 0; }
 
 void mhd_ensemble_destroy(mhd_ensemble_t* e)
 {
@@ -1758,7 +1896,8 @@ int mhd_closure_attach(mhd_ctx_t* c, mhd_closure_t* cl)
     if (!c->cl_mem) {
         const size_t fw = (size_t)c->n3 * (c->wf32 ? 4 : 8);
         const size_t fa = (fw + 255) & ~(size_t)255;
-        if (hipMalloc(&c->cl_mem, 24 * fa + (size_t)c->nz * 8) != hipSuccess) {
+        const size_t ga = (closure_grid_bytes(c->n3) + 255) & ~(size_t)255;
+        if (hipMalloc(&c->cl_mem, 24 * fa + ga + (size_t)c->nz * 8) != hipSuccess) {
             c->cl_mem = nullptr;
             return MHD_E_ALLOC;
         }
@@ -1767,12 +1906,36 @@ int mhd_closure_attach(mhd_ctx_t* c, mhd_closure_t* cl)
         for (int m = 0; m < 4; ++m, p += fa) c->cdx[m] = p;
         for (int m = 0; m < 4; ++m, p += fa) c->cdy[m] = p;
         for (int m = 0; m < 4; ++m, p += fa) c->cdz[m] = p;
+        c->cgrid = p;   // this run's feature grid: one closure may serve several runs
+        p += ga;
         c->cneg1 = (double*)p;
         std::vector<double> neg(c->nz, -1.0);
         MHD_HIP_CHECK(hipMemcpy(c->cneg1, neg.data(), (size_t)c->nz * 8, hipMemcpyHostToDevice));
     }
     c->closure = cl;
     return 0;
+}
+
+int mhd_closure_add_state(mhd_closure_t* cl, mhd_ctx_t* fine, int f)
+{
+    if (!cl) return MHD_E_ARG;
+    TRY(check_ctx(fine));
+    if (!fine->have_state) return MHD_E_STATE;
+    TRY(mhd_sync(fine));   // the state is complete before the closure's stream reads it
+    ClSource src;
+    memset(&src, 0, sizeof(src));
+    const double* f64[5] = {fine->r, fine->t, fine->u, fine->v, fine->w};
+    for (int q = 0; q < 5; ++q) src.p[q] = f64[q];
+    src.p[5] = fine->bx;
+    src.p[6] = fine->by;
+    src.p[7] = fine->bz;
+    for (int q = 5; q < 8; ++q) src.f32[q] = fine->wf32;
+    if (fine->su) {
+        src.mean[5] = fine->bxbar;
+        src.mean[6] = fine->bybar;
+        src.add[7] = fine->sc.bzbar;
+    }
+    return closure_add_source(cl, src, fine->nx, fine->ny, fine->nz, f, fine->p.f);
 }
 
 int mhd_global_averages(mhd_ctx_t* c, double out[4])

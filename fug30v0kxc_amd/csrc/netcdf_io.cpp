@@ -2,14 +2,15 @@
 // and read without libnetcdf.
 //
 // Format: netCDF classic (CDF-1, 32-bit offsets, big-endian), as created by
-// the reference's CreateNetCDF / WriteNetCDF (src/netcdf.F90:39-580) with
+// the reference's CreateNetCDF / WriteNetCDF (src/netcdf.F90:39-580), or its
+// 64-bit-offset variant (CDF-2) when a variable begins past 2 GiB, with
 // itype = 1: dimensions nx, ny, nz, one; global attributes version (int),
 // history (text), gamma, sigma, theta, ck, cm, f, tayl, psi, zeta, xmax, ymax
 // (double); variables time(one) double with units="UNKNOWN",
 // iteration(one) int, timestep(one) double, then R, T, U, V, W, Bx, By, Bz
 // over (nz, ny, nx) as double (iprec 8) or float (iprec 4).  The reader
-// accepts any classic file holding those variables (double or float) and
-// any attribute/dimension order.
+// accepts any CDF-1 or CDF-2 file holding those variables (double or float)
+// and any attribute/dimension order.
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -132,98 +133,135 @@ double be_f64(const unsigned char* q)
     memcpy(&d, &v, 8);
     return d;
 }
-float be_f32(const unsigned char* q)
+
+// The header of a dump: dimensions, global attributes and the variables with
+// their begin offsets.  CDF-1 (32-bit signed offsets) when every variable
+// begins below 2^31, as the reference's nf_create(NF_CLOBBER) writes;
+// otherwise CDF-2 (64-bit offsets), which libnetcdf and scipy read as well.
+// Returns 0, or MHD_E_ARG when a variable exceeds the format's 4 GiB limit.
+int build_header(const mhd_dump_t* meta, int iprec, Out& h, int* version_out)
 {
-    uint32_t v = ((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3];
-    float f;
-    memcpy(&f, &v, 4);
-    return f;
+    const int nx = meta->nx, ny = meta->ny, nz = meta->nz;
+    const size_t n3 = (size_t)nx * ny * nz;
+    const int ftype = iprec == 8 ? NC_DOUBLE : NC_FLOAT;
+    const size_t fsize = (size_t)iprec * n3;
+    if (fsize > 0xfffffffcULL) return MHD_E_ARG;   // fixed-size variables of CDF-1/2
+    for (int version = 1; version <= 2; ++version) {
+        h.b.clear();
+        h.b.insert(h.b.end(), {'C', 'D', 'F', (unsigned char)version});
+        h.u32(0);  // numrecs (no record dimension)
+        // dimensions
+        h.u32(NC_DIMENSION);
+        h.u32(4);
+        const char* dn[4] = {"nx", "ny", "nz", "one"};
+        const int dl[4] = {nx, ny, nz, 1};
+        for (int i = 0; i < 4; ++i) {
+            h.name(dn[i]);
+            h.u32((uint32_t)dl[i]);
+        }
+        // global attributes
+        const double* pv = &meta->gamma;
+        h.u32(NC_ATTRIBUTE);
+        h.u32(13);
+        h.name("version");
+        h.u32(NC_INT);
+        h.u32(1);
+        h.i32(meta->version);
+        std::string hist(meta->history, strnlen(meta->history, sizeof(meta->history)));
+        h.name("history");
+        h.u32(NC_CHAR);
+        h.u32((uint32_t)hist.size());
+        h.b.insert(h.b.end(), hist.begin(), hist.end());
+        h.pad4();
+        for (int i = 0; i < 11; ++i) {
+            h.name(kAttrs[i]);
+            h.u32(NC_DOUBLE);
+            h.u32(1);
+            h.f64(pv[i]);
+        }
+        // variables: sizes and offsets follow the header, so lay the header
+        // out with placeholder offsets first and patch them
+        struct V { const char* name; int ndim; int dims[3]; int type; size_t vsize; bool units; };
+        std::vector<V> vars = {{"time", 1, {3, 0, 0}, NC_DOUBLE, 8, true},
+                               {"iteration", 1, {3, 0, 0}, NC_INT, 4, false},
+                               {"timestep", 1, {3, 0, 0}, NC_DOUBLE, 8, false}};
+        for (int f = 0; f < 8; ++f)
+            vars.push_back({kFields[f], 3, {2, 1, 0}, ftype, (fsize + 3) & ~(size_t)3, false});
+        h.u32(NC_VARIABLE);
+        h.u32((uint32_t)vars.size());
+        std::vector<size_t> begin_pos;
+        for (const V& v : vars) {
+            h.name(v.name);
+            h.u32((uint32_t)v.ndim);
+            for (int d = 0; d < v.ndim; ++d) h.u32((uint32_t)v.dims[d]);
+            if (v.units) {
+                h.u32(NC_ATTRIBUTE);
+                h.u32(1);
+                h.name("units");
+                h.u32(NC_CHAR);
+                h.u32(7);
+                h.b.insert(h.b.end(), {'U', 'N', 'K', 'N', 'O', 'W', 'N'});
+                h.pad4();
+            } else {
+                h.u32(0);
+                h.u32(0);
+            }
+            h.u32((uint32_t)v.type);
+            h.u32((uint32_t)v.vsize);
+            begin_pos.push_back(h.b.size());
+            h.u32(0);
+            if (version == 2) h.u32(0);
+        }
+        size_t off = h.b.size();
+        bool fits = true;
+        for (size_t i = 0; i < vars.size(); ++i) {
+            if (version == 1) {
+                if (off > 0x7fffffffULL) { fits = false; break; }   // signed 32-bit offsets
+                for (int s = 0; s < 4; ++s) h.b[begin_pos[i] + s] = (unsigned char)(off >> (24 - 8 * s));
+            } else {
+                for (int s = 0; s < 8; ++s) h.b[begin_pos[i] + s] = (unsigned char)((uint64_t)off >> (56 - 8 * s));
+            }
+            off += vars[i].vsize;
+        }
+        if (fits) {
+            *version_out = version;
+            return 0;
+        }
+    }
+    return MHD_E_ARG;
 }
 
 }  // namespace
 
 extern "C" {
 
+long mhd_dump_header(const mhd_dump_t* meta, int iprec, unsigned char* buf, long cap, int* version)
+{
+    if (!meta || (iprec != 4 && iprec != 8) || meta->nx < 1 || meta->ny < 1 || meta->nz < 1) return MHD_E_ARG;
+    Out h;
+    int v = 0;
+    const int e = build_header(meta, iprec, h, &v);
+    if (e) return e;
+    if (version) *version = v;
+    if (buf && cap >= (long)h.b.size()) memcpy(buf, h.b.data(), h.b.size());
+    return (long)h.b.size();
+}
+
 int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const fields[8], int iprec)
 {
     if (!path || !meta || !fields || (iprec != 4 && iprec != 8)) return MHD_E_ARG;
     const int nx = meta->nx, ny = meta->ny, nz = meta->nz;
+    if (nx < 1 || ny < 1 || nz < 1) return MHD_E_ARG;
     const size_t n3 = (size_t)nx * ny * nz;
     for (int f = 0; f < 8; ++f)
         if (!fields[f]) return MHD_E_ARG;
     Out h;
-    h.b.insert(h.b.end(), {'C', 'D', 'F', 1});
-    h.u32(0);  // numrecs (no record dimension)
-    // dimensions
-    h.u32(NC_DIMENSION);
-    h.u32(4);
-    const char* dn[4] = {"nx", "ny", "nz", "one"};
-    const int dl[4] = {nx, ny, nz, 1};
-    for (int i = 0; i < 4; ++i) {
-        h.name(dn[i]);
-        h.u32((uint32_t)dl[i]);
+    int version = 0;
+    {
+        const int e = build_header(meta, iprec, h, &version);
+        if (e) return e;
     }
-    // global attributes
-    const double* pv = &meta->gamma;
-    h.u32(NC_ATTRIBUTE);
-    h.u32(13);
-    h.name("version");
-    h.u32(NC_INT);
-    h.u32(1);
-    h.i32(meta->version);
-    std::string hist(meta->history);
-    h.name("history");
-    h.u32(NC_CHAR);
-    h.u32((uint32_t)hist.size());
-    h.b.insert(h.b.end(), hist.begin(), hist.end());
-    h.pad4();
-    for (int i = 0; i < 11; ++i) {
-        h.name(kAttrs[i]);
-        h.u32(NC_DOUBLE);
-        h.u32(1);
-        h.f64(pv[i]);
-    }
-    // variables: sizes and offsets follow the header, so lay the header out
-    // with placeholder offsets first and patch them
-    const int ftype = iprec == 8 ? NC_DOUBLE : NC_FLOAT;
     const size_t fsize = (size_t)iprec * n3;
-    struct V { const char* name; int ndim; int dims[3]; int type; size_t vsize; bool units; };
-    std::vector<V> vars = {{"time", 1, {3, 0, 0}, NC_DOUBLE, 8, true},
-                           {"iteration", 1, {3, 0, 0}, NC_INT, 4, false},
-                           {"timestep", 1, {3, 0, 0}, NC_DOUBLE, 8, false}};
-    for (int f = 0; f < 8; ++f) vars.push_back({kFields[f], 3, {2, 1, 0}, ftype, (fsize + 3) & ~(size_t)3, false});
-    if (fsize > 0x7fffffffULL) return MHD_E_ARG;  // CDF-1 variable size limit
-    h.u32(NC_VARIABLE);
-    h.u32((uint32_t)vars.size());
-    std::vector<size_t> begin_pos;
-    for (const V& v : vars) {
-        h.name(v.name);
-        h.u32((uint32_t)v.ndim);
-        for (int d = 0; d < v.ndim; ++d) h.u32((uint32_t)v.dims[d]);
-        if (v.units) {
-            h.u32(NC_ATTRIBUTE);
-            h.u32(1);
-            h.name("units");
-            h.u32(NC_CHAR);
-            h.u32(7);
-            h.b.insert(h.b.end(), {'U', 'N', 'K', 'N', 'O', 'W', 'N'});
-            h.pad4();
-        } else {
-            h.u32(0);
-            h.u32(0);
-        }
-        h.u32((uint32_t)v.type);
-        h.u32((uint32_t)(v.vsize > 0xffffffffULL ? 0xffffffffULL : v.vsize));
-        begin_pos.push_back(h.b.size());
-        h.u32(0);
-    }
-    size_t off = h.b.size();
-    for (size_t i = 0; i < vars.size(); ++i) {
-        if (off > 0xffffffffULL) return MHD_E_ARG;  // beyond 32-bit offsets
-        const uint32_t b = (uint32_t)off;
-        for (int s = 0; s < 4; ++s) h.b[begin_pos[i] + s] = (unsigned char)(b >> (24 - 8 * s));
-        off += vars[i].vsize;
-    }
     // fields converted to big-endian on the host threads, one buffer per
     // field (a mapped output file measured slower on the GPU hosts)
     FILE* fp = fopen(path, "wb");
@@ -301,7 +339,8 @@ int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8])
     int e = map_file(path, data);
     if (e) return e;
     In in{data.data(), data.size()};
-    if (memcmp(data.data(), "CDF\x01", 4) != 0) return -15;
+    if (memcmp(data.data(), "CDF", 3) != 0 || (data.data()[3] != 1 && data.data()[3] != 2)) return -15;
+    const bool off64 = data.data()[3] == 2;   // CDF-2: 64-bit begin offsets
     in.o = 4;
     in.u32();  // numrecs
     std::vector<std::string> dname;
@@ -360,6 +399,7 @@ int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8])
         int type = (int)in.u32();
         in.u32();  // vsize
         size_t begin = in.u32();
+        if (off64) begin = (begin << 32) | in.u32();
         if (!in.ok || begin + 8 > data.size()) return -15;
         const unsigned char* q = data.data() + begin;
         if (nm == "time" && type == NC_DOUBLE) meta->time = be_f64(q);

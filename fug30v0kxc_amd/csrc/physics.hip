@@ -2,6 +2,7 @@
 // spectral operations, AB3 update, boundary conditions and reductions for
 // gfx950.  Point-wise kernels use one thread per grid point with x fastest,
 // so every wavefront reads 64 consecutive elements of each operand.
+#include "launch.hpp"
 #include "physics.hpp"
 #include "spectral.hpp"
 #include "zstencil.hpp"
@@ -24,13 +25,20 @@ constexpr int ZKC = 16;
 // TC: compute/output type; TIN: input element type; KIND: ZKind; PROD: the
 // input is scale*in*in1.  w (upwind selector) has the element type TW.
 template <typename TC, typename TIN, typename TW, int KIND, bool PROD>
-__global__ void __launch_bounds__(NT) z_kernel(ZLaunch Lv)
+struct ZKernel {
+    using Args = ZLaunch;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const ZLaunch& L, uint3 bid, uint3 gdim);
+};
+
+template <typename TC, typename TIN, typename TW, int KIND, bool PROD>
+__device__ __forceinline__ void ZKernel<TC, TIN, TW, KIND, PROD>::body(const ZLaunch& L, uint3 bid, uint3 gdim)
 {
-    (void)Lv;
-    const ZLaunch& L = kernarg<ZLaunch>();
-    const ZJob& J = L.job[blockIdx.z];
+    (void)bid;
+    (void)gdim;
+    const ZJob& J = L.job[bid.z];
     const int nxy = L.g.nxy, nz = L.g.nz;
-    const long col = (long)blockIdx.x * NT + threadIdx.x;
+    const long col = (long)bid.x * NT + threadIdx.x;
     if (col >= nxy) return;
     const TIN* __restrict__ in = (const TIN*)J.in;
     const TIN* __restrict__ in1 = (const TIN*)J.in1;
@@ -45,7 +53,7 @@ __global__ void __launch_bounds__(NT) z_kernel(ZLaunch Lv)
         if constexpr (PROD) v = (scale * v) * (TC)in1[i];
         return v;
     };
-    const int k0 = blockIdx.y * ZKC;
+    const int k0 = bid.y * ZKC;
     if (k0 >= 3 && k0 + ZKC <= nz - 3) {
         constexpr int LO = (KIND == ZK_D1UP) ? 0 : 1;          // first window slot used
         constexpr int HI = (KIND == ZK_D1UP) ? ZKC + 6 : ZKC + 5;
@@ -88,8 +96,7 @@ template <typename TC, typename TIN, typename TW, int KIND, bool PROD>
 static int launch_z_group(ZLaunch& G, hipStream_t s)
 {
     dim3 grid(nblk(G.g.nxy), (unsigned)((G.g.nz + ZKC - 1) / ZKC), G.njobs);
-    hipLaunchKernelGGL((z_kernel<TC, TIN, TW, KIND, PROD>), grid, dim3(NT), 0, s, G);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
+    return mhd_launch<ZKernel<TC, TIN, TW, KIND, PROD>>(grid, dim3(NT), 0, s, G);
 }
 
 template <typename TC, typename TIN, typename TW>
@@ -182,12 +189,21 @@ __device__ __forceinline__ bool in_band(int p, int q, int nx, int ny, int icx, i
 // src/magnetic.f90:362-407: cc = dEx/dy - dEy/dx, d = dEx/dx + dEy/dy by index
 // swaps, one thread per mirror quad as in bandj_spec_kernel
 template <typename TW>
-__global__ void __launch_bounds__(NT) mag_spec1_kernel(MagArgs a)
+struct MagSpec1Kernel {
+    using Args = MagArgs;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const MagArgs& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TW>
+__device__ __forceinline__ void MagSpec1Kernel<TW>::body(const MagArgs& a, uint3 bid, uint3 gdim)
 {
+    (void)bid;
+    (void)gdim;
     const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
     const int hx = nx / 2 + 1, hy = ny / 2 + 1;
     const long perk = (long)hx * hy;
-    const long t = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+    const long t = xcd_tile(bid.x, gdim.x) * NT + threadIdx.x;
     if (t >= perk * a.g.nz) return;
     const int k = (int)(t / perk);
     const int rem = (int)(t - (long)k * perk);
@@ -232,8 +248,17 @@ __global__ void __launch_bounds__(NT) mag_spec1_kernel(MagArgs a)
 
 // mean-field equations, src/magnetic.f90:328-352 (one workgroup)
 template <typename TW>
-__global__ void __launch_bounds__(1024) mean_emf_kernel(MagArgs a)
+struct MeanEmfKernel {
+    using Args = MagArgs;
+    static constexpr int NT = 1024;
+    static __device__ __forceinline__ void body(const MagArgs& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TW>
+__device__ __forceinline__ void MeanEmfKernel<TW>::body(const MagArgs& a, uint3 bid, uint3 gdim)
 {
+    (void)bid;
+    (void)gdim;
     extern __shared__ double sh[];
     const int nz = a.g.nz;
     double* w1d = sh;
@@ -264,18 +289,25 @@ __global__ void __launch_bounds__(1024) mean_emf_kernel(MagArgs a)
 // so each plane is read once; the two planes next to each wall use the
 // one-sided d2bydz rows (flag 3) with direct loads.
 template <typename TW>
-__global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
+struct MagRhsKernel {
+    using Args = MagArgs;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const MagArgs& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TW>
+__device__ __forceinline__ void MagRhsKernel<TW>::body(const MagArgs& a, uint3 bid, uint3 gdim)
 {
-    (void)av;
-    const MagArgs& a = kernarg<MagArgs>();
+    (void)bid;
+    (void)gdim;
     using TW2 = TW __attribute__((ext_vector_type(2)));
     using D2 = double __attribute__((ext_vector_type(2)));
     const int nx = a.g.nx, nxy = a.g.nxy, nz = a.g.nz;
     // a thread owns two neighbouring columns of one row (nx is even): every
     // load and store moves two elements
-    const long col = 2 * ((long)blockIdx.x * NT + threadIdx.x);
+    const long col = 2 * ((long)bid.x * NT + threadIdx.x);
     if (col >= nxy) return;
-    const int k0 = blockIdx.y * a.zc;
+    const int k0 = bid.y * a.zc;
     const int k1 = (k0 + a.zc < nz) ? k0 + a.zc : nz;
     const int jq = (int)(col / nx), ip0 = (int)(col - (long)jq * nx);
     double k2[2];
@@ -380,12 +412,21 @@ __global__ void __launch_bounds__(NT) mag_rhs_kernel(MagArgs av)
 // mirrors), reads pol, tor and d pol/dz once at each of its points and writes
 // all six outputs there, with the arithmetic of the per-point formulas.
 template <typename TW>
-__global__ void __launch_bounds__(NT) bandj_spec_kernel(MagArgs a)
+struct BandjSpecKernel {
+    using Args = MagArgs;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const MagArgs& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TW>
+__device__ __forceinline__ void BandjSpecKernel<TW>::body(const MagArgs& a, uint3 bid, uint3 gdim)
 {
+    (void)bid;
+    (void)gdim;
     const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
     const int hx = nx / 2 + 1, hy = ny / 2 + 1;
     const long perk = (long)hx * hy;
-    const long t = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+    const long t = xcd_tile(bid.x, gdim.x) * NT + threadIdx.x;
     if (t >= perk * a.g.nz) return;
     const int k = (int)(t / perk);
     const int rem = (int)(t - (long)k * perk);
@@ -444,15 +485,22 @@ __global__ void __launch_bounds__(NT) bandj_spec_kernel(MagArgs a)
 // thread owns two neighbouring columns (nx is even), so every load and store
 // moves two elements.
 template <typename TW>
-__global__ void __launch_bounds__(NT) jsq_kernel(MagArgs av)
+struct JsqKernel {
+    using Args = MagArgs;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const MagArgs& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TW>
+__device__ __forceinline__ void JsqKernel<TW>::body(const MagArgs& a, uint3 bid, uint3 gdim)
 {
-    (void)av;
-    const MagArgs& a = kernarg<MagArgs>();
+    (void)bid;
+    (void)gdim;
     using TW2 = TW __attribute__((ext_vector_type(2)));
     const int nxy = a.g.nxy, nz = a.g.nz;
-    const long col = 2 * ((long)blockIdx.x * NT + threadIdx.x);
+    const long col = 2 * ((long)bid.x * NT + threadIdx.x);
     if (col >= nxy) return;
-    const int k0 = blockIdx.y * a.zc;
+    const int k0 = bid.y * a.zc;
     const int k1 = (k0 + a.zc < nz) ? k0 + a.zc : nz;
     const TW* by = (const TW*)a.by;
     const TW* bx = (const TW*)a.bx;
@@ -527,9 +575,18 @@ __global__ void __launch_bounds__(NT) jsq_kernel(MagArgs av)
 
 // mag_init spectral part, src/magnetic.f90:63-233 (truncation folded into reads)
 template <typename TW>
-__global__ void __launch_bounds__(NT) mag_init_spec_kernel(MagArgs a)
+struct MagInitSpecKernel {
+    using Args = MagArgs;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const MagArgs& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TW>
+__device__ __forceinline__ void MagInitSpecKernel<TW>::body(const MagArgs& a, uint3 bid, uint3 gdim)
 {
-    const long q = xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+    (void)bid;
+    (void)gdim;
+    const long q = xcd_tile(bid.x, gdim.x) * NT + threadIdx.x;
     if (q >= a.g.n3) return;
     const int nx = a.g.nx, ny = a.g.ny, nxy = a.g.nxy;
     const int k = (int)(q / nxy);
@@ -587,47 +644,47 @@ __global__ void bzbar_kernel(MagArgs a, int wf32, double* out)
     out[0] = v * a.p.xn;
 }
 
-#define MHD_LAUNCH_W(kern, grid, block, shm, s, arg)                                       \
+#define MHD_LAUNCH_W(K, grid, block, shm, s, arg)                                          \
     do {                                                                                   \
-        if ((arg).wf32) hipLaunchKernelGGL(kern<float>, grid, block, shm, s, arg);         \
-        else hipLaunchKernelGGL(kern<double>, grid, block, shm, s, arg);                   \
-        if (hipGetLastError() != hipSuccess) return -22;                                   \
+        const int e_ = (arg).wf32 ? mhd_launch<K<float>>(grid, block, shm, s, arg)         \
+                                  : mhd_launch<K<double>>(grid, block, shm, s, arg);       \
+        if (e_) return -22;                                                                \
     } while (0)
 
 int launch_mag_spec1(const MagArgs& a, hipStream_t s)
 {
     const long quads = (long)(a.g.nx / 2 + 1) * (a.g.ny / 2 + 1) * a.g.nz;
-    MHD_LAUNCH_W(mag_spec1_kernel, dim3(nblk(quads)), dim3(NT), 0, s, a);
+    MHD_LAUNCH_W(MagSpec1Kernel, dim3(nblk(quads)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_mean_emf(const MagArgs& a, hipStream_t s)
 {
-    MHD_LAUNCH_W(mean_emf_kernel, dim3(1), dim3(256), 4 * a.g.nz * sizeof(double), s, a);
+    MHD_LAUNCH_W(MeanEmfKernel, dim3(1), dim3(256), 4 * a.g.nz * sizeof(double), s, a);
     return 0;
 }
 int launch_mag_rhs(const MagArgs& a, hipStream_t s)
 {
     if (a.zc <= 0) return -10;
     if (a.g.nxy % 2) return -10;   // two columns per thread (nx is even)
-    MHD_LAUNCH_W(mag_rhs_kernel, dim3(nblk(a.g.nxy / 2), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
+    MHD_LAUNCH_W(MagRhsKernel, dim3(nblk(a.g.nxy / 2), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_bandj_spec(const MagArgs& a, hipStream_t s)
 {
     const long quads = (long)(a.g.nx / 2 + 1) * (a.g.ny / 2 + 1) * a.g.nz;
-    MHD_LAUNCH_W(bandj_spec_kernel, dim3(nblk(quads)), dim3(NT), 0, s, a);
+    MHD_LAUNCH_W(BandjSpecKernel, dim3(nblk(quads)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_jsq(const MagArgs& a, hipStream_t s)
 {
     if (a.zc <= 0) return -10;
     if (a.g.nxy % 2) return -10;   // two columns per thread (nx is even)
-    MHD_LAUNCH_W(jsq_kernel, dim3(nblk(a.g.nxy / 2), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
+    MHD_LAUNCH_W(JsqKernel, dim3(nblk(a.g.nxy / 2), (unsigned)((a.g.nz + a.zc - 1) / a.zc)), dim3(NT), 0, s, a);
     return 0;
 }
 int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out)
 {
-    MHD_LAUNCH_W(mag_init_spec_kernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
+    MHD_LAUNCH_W(MagInitSpecKernel, dim3(nblk(a.g.n3)), dim3(NT), 0, s, a);
     hipLaunchKernelGGL(bzbar_kernel, dim3(1), dim3(1), 0, s, a, a.wf32, means_out);
     return hipGetLastError() == hipSuccess ? 0 : -22;
 }
@@ -636,11 +693,18 @@ int launch_mag_init_spec(const MagArgs& a, hipStream_t s, double* means_out)
 // AB3 update, src/main.F90:413-477, plus magdealias (src/magnetic.f90:666-717)
 // ---------------------------------------------------------------------------
 template <typename TH>
-__global__ void __launch_bounds__(NT) ab3_kernel(AB3Args av)
+struct AB3Kernel {
+    using Args = AB3Args;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const AB3Args& a, uint3 bid, uint3 gdim);
+};
+
+template <typename TH>
+__device__ __forceinline__ void AB3Kernel<TH>::body(const AB3Args& a, uint3 bid, uint3 gdim)
 {
-    (void)av;
-    const AB3Args& a = kernarg<AB3Args>();
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
+    (void)bid;
+    (void)gdim;
+    const long q = (long)bid.x * NT + threadIdx.x;
     if (q < a.g.n3) {
         const double c0 = a.c0, c1 = a.c1, c2 = a.c2;
         // every entry defined; field f is updated when f0 <= f < nfields
@@ -694,9 +758,8 @@ int launch_ab3(const AB3Args& a, hipStream_t s)
 {
     // with no field to update (f0 = 7) only the nz means are left
     long n = a.f0 < a.nfields ? (a.g.n3 > a.g.nz ? a.g.n3 : a.g.nz) : a.g.nz;
-    if (a.h32) hipLaunchKernelGGL(ab3_kernel<float>, dim3(nblk(n)), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL(ab3_kernel<double>, dim3(nblk(n)), dim3(NT), 0, s, a);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
+    return a.h32 ? mhd_launch<AB3Kernel<float>>(dim3(nblk(n)), dim3(NT), 0, s, a)
+                 : mhd_launch<AB3Kernel<double>>(dim3(nblk(n)), dim3(NT), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -705,11 +768,17 @@ int launch_ab3(const AB3Args& a, hipStream_t s)
 // options; vertical field (4660-4750) or perfect conductors (4314-4400) for
 // pol, tor and the means.  elf = 1/11.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NT) boundary_kernel(BCArgs av)
+struct BoundaryKernel {
+    using Args = BCArgs;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const BCArgs& a, uint3 bid, uint3 gdim);
+};
+
+__device__ __forceinline__ void BoundaryKernel::body(const BCArgs& a, uint3 bid, uint3 gdim)
 {
-    (void)av;
-    const BCArgs& a = kernarg<BCArgs>();
-    const long p = (long)blockIdx.x * NT + threadIdx.x;
+    (void)bid;
+    (void)gdim;
+    const long p = (long)bid.x * NT + threadIdx.x;
     const long m = a.g.nxy;
     const double elf = 1.0 / 11.0;
     if (p < m && a.hydro) {
@@ -786,8 +855,7 @@ __global__ void __launch_bounds__(NT) boundary_kernel(BCArgs av)
 
 int launch_boundary(const BCArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(boundary_kernel, dim3(nblk(a.g.nxy)), dim3(NT), 0, s, a);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
+    return mhd_launch<BoundaryKernel>(dim3(nblk(a.g.nxy)), dim3(NT), 0, s, a);
 }
 
 __global__ void __launch_bounds__(NT) magdealias_kernel(Geom g, double* pol, double* tor, int icx, int icy)
@@ -812,26 +880,42 @@ int launch_magdealias(Geom g, double* pol, double* tor, int icutx, int icuty, hi
 // reductions: one workgroup per plane writes partial[k][0..7]; one finishing
 // workgroup combines the planes in k order like the reference's loops
 // ---------------------------------------------------------------------------
+// Workgroup reduction of NV values (kind: 0 sum, 1 min, 2 max).  Within each
+// 64-wide wavefront a butterfly of cross-lane moves (__shfl_xor, no LDS)
+// leaves every lane with the wave's result; the NT / 64 wave results then
+// pass through LDS and thread 0 combines them in wave order.  The order of
+// combination is fixed, so results do not depend on timing.  The result is
+// valid in thread 0.
 template <int NV>
 __device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&kind)[NV], double* sh)
 {
-    // kind: 0 sum, 1 min, 2 max
-    const int tid = threadIdx.x;
+    static_assert(NT % 64 == 0, "whole wavefronts");
+    constexpr int NW = NT / 64;
 #pragma unroll
-    for (int i = 0; i < NV; ++i) sh[i * NT + tid] = v[i];
-    __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if (tid < st) {
+    for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                const double a = sh[i * NT + tid], b = sh[i * NT + tid + st];
-                sh[i * NT + tid] = kind[i] == 0 ? a + b : kind[i] == 1 ? fmin(a, b) : fmax(a, b);
-            }
+        for (int i = 0; i < NV; ++i) {
+            const double o = __shfl_xor(v[i], off, 64);
+            v[i] = kind[i] == 0 ? v[i] + o : kind[i] == 1 ? fmin(v[i], o) : fmax(v[i], o);
         }
-        __syncthreads();
     }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = sh[i * NT];
+        for (int i = 0; i < NV; ++i) sh[i * NW + wave] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double x = sh[i * NW];
+            for (int w = 1; w < NW; ++w) {
+                const double b = sh[i * NW + w];
+                x = kind[i] == 0 ? x + b : kind[i] == 1 ? fmin(x, b) : fmax(x, b);
+            }
+            v[i] = x;
+        }
+    }
 }
 
 template <typename TW>
@@ -840,7 +924,7 @@ __device__ __forceinline__ double rdw(const void* p, long q) { return (double)((
 template <typename TW>
 __global__ void __launch_bounds__(NT) reduce_plane_kernel(RedArgs a)
 {
-    __shared__ double sh[5 * NT];
+    __shared__ double sh[5 * (NT / 64)];
     // workgroup = (plane k, chunk ch): the points [lo, hi) of the plane
     const int k = blockIdx.x / RED_CHUNKS, ch = blockIdx.x - k * RED_CHUNKS;
     const long base = (long)k * a.g.nxy;
@@ -1043,28 +1127,38 @@ int launch_com_shift(Geom g, double* u, double* v, const double* res, hipStream_
 // ---------------------------------------------------------------------------
 // mixed precision: FP32 fluctuation shadows of the FP64 state
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NT) shadow_kernel(Geom g, const double* u, const double* v, const double* w,
-                                                    const double* t, const double* r, float* su, float* sv,
-                                                    float* sw, float* st, float* sr, const double* t0,
-                                                    const double* r0)
-{
-    const long q = (long)blockIdx.x * NT + threadIdx.x;
-    if (q >= g.n3) return;
-    const int k = (int)(q / g.nxy);
-    su[q] = (float)u[q];
-    sv[q] = (float)v[q];
-    sw[q] = (float)w[q];
-    st[q] = (float)(t[q] - t0[k]);
-    sr[q] = (float)(r[q] - r0[k]);
-}
+// FP32 fluctuation shadows of the FP64 state (mixed precision)
+struct ShadowArgs {
+    Geom g;
+    const double *u, *v, *w, *t, *r, *t0, *r0;
+    float *su, *sv, *sw, *st, *sr;
+};
+struct ShadowKernel {
+    using Args = ShadowArgs;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void body(const ShadowArgs& a, uint3 bid, uint3 gdim)
+    {
+        (void)gdim;
+        const long q = (long)bid.x * NT + threadIdx.x;
+        if (q >= a.g.n3) return;
+        const int k = (int)(q / a.g.nxy);
+        a.su[q] = (float)a.u[q];
+        a.sv[q] = (float)a.v[q];
+        a.sw[q] = (float)a.w[q];
+        a.st[q] = (float)(a.t[q] - a.t0[k]);
+        a.sr[q] = (float)(a.r[q] - a.r0[k]);
+    }
+};
 
 int launch_shadow(Geom g, const double* u, const double* v, const double* w, const double* t, const double* r,
                   float* su, float* sv, float* sw, float* st, float* sr, const double* t0, const double* r0,
                   hipStream_t s)
 {
-    hipLaunchKernelGGL(shadow_kernel, dim3(nblk(g.n3)), dim3(NT), 0, s, g, u, v, w, t, r, su, sv, sw, st, sr, t0,
-                       r0);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
+    ShadowArgs a;
+    a.g = g;
+    a.u = u; a.v = v; a.w = w; a.t = t; a.r = r; a.t0 = t0; a.r0 = r0;
+    a.su = su; a.sv = sv; a.sw = sw; a.st = st; a.sr = sr;
+    return mhd_launch<ShadowKernel>(dim3(nblk(g.n3)), dim3(NT), 0, s, a);
 }
 
 __global__ void __launch_bounds__(NT) add_mean_kernel(const float* src, double* dst, const double* mean, double sc,

@@ -12,7 +12,6 @@
 // a factor 2 against the transform's 1/(2n)).
 #include <cmath>
 #include <cstring>
-#include <vector>
 
 #include "../../include/mhd.h"
 #include "physics.hpp"
@@ -78,22 +77,16 @@ __global__ void __launch_bounds__(RNT) box_kernel(const double* __restrict__ in,
 
 unsigned nb(long n) { return (unsigned)((n + RNT - 1) / RNT); }
 
-// FP64 twiddles W_n^t = exp(-2 pi i t / n) on the device
-double* twiddles64(int n)
+// W_n^t = exp(-2 pi i t / n) for t < n, as (cos, sin) pairs, computed on the
+// device so that no host staging is needed
+__global__ void __launch_bounds__(RNT) twiddle_kernel(double* __restrict__ tw, int n)
 {
-    std::vector<double> h(2 * (size_t)n);
-    for (int t = 0; t < n; ++t) {
-        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
-        h[2 * t] = (double)cosl(a);
-        h[2 * t + 1] = (double)sinl(a);
-    }
-    double* d = nullptr;
-    if (hipMalloc((void**)&d, h.size() * sizeof(double)) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(d);
-        return nullptr;
-    }
-    return d;
+    const int t = blockIdx.x * RNT + threadIdx.x;
+    if (t >= n) return;
+    double s, c;
+    sincospi(-2.0 * (double)t / (double)n, &s, &c);
+    tw[2 * t] = c;
+    tw[2 * t + 1] = s;
 }
 
 int ncus()
@@ -105,8 +98,10 @@ int ncus()
     return n;
 }
 
-// one in-place real <-> halfcomplex pass over a (nx, ny, nz) FP64 box
-int hc_pass(double* a, int nx, int ny, int nz, int dir, int mode, const double* tw, hipStream_t s)
+// one real <-> halfcomplex pass over a (nx, ny, nz) FP64 box, from `in` to
+// `out` (the same array for an in-place pass)
+int hc_pass(const double* in, double* out, int nx, int ny, int nz, int dir, int mode, const double* tw,
+            hipStream_t s)
 {
     SpecLaunch L;
     memset(&L, 0, sizeof(L));
@@ -117,12 +112,57 @@ int hc_pass(double* a, int nx, int ny, int nz, int dir, int mode, const double* 
     L.tw = tw;
     L.ncu = ncus();
     SpecJob& J = L.job[0];
-    J.in[0] = a;
-    J.out[0] = a;
+    J.in[0] = in;
+    J.out[0] = out;
     J.load_op = LD_PLAIN;
     J.scale = 1.0;
     const int e = launch_spectral(L, 0, s);
     return e == -20 ? MHD_E_SIZE : (e ? MHD_E_DEVICE : 0);
+}
+
+// device doubles expand needs besides its input and output: the x-padded box
+// (2 nx ny nz) and the four twiddle tables (lengths nx, 2nx, ny, 2ny)
+size_t expand_work_doubles(int nx, int ny, int nz)
+{
+    return 2 * (size_t)nx * ny * nz + 2 * (size_t)(3 * nx + 3 * ny);
+}
+
+int expand_check(int nx, int ny, int nz, const void* in, const void* out)
+{
+    if (!in || !out || nz < 1 || nx < 2 || ny < 2) return MHD_E_ARG;
+    if (nx % 2) return MHD_E_ODD_NX;
+    if (ny % 2) return MHD_E_ODD_NY;
+    if (!spectral_len_supported(nx) || !spectral_len_supported(2 * nx) || !spectral_len_supported(ny) ||
+        !spectral_len_supported(2 * ny))
+        return MHD_E_SIZE;
+    return 0;
+}
+
+// expand of device arrays: in (nx, ny, nz) -> out (2nx, 2ny, nz), with
+// `work` holding expand_work_doubles() doubles; `in` is only read.  The x
+// forward pass writes the first quarter of `out`, which the y padding later
+// overwrites.
+int expand_device(int nx, int ny, int nz, const double* in, double* out, double* work, hipStream_t s)
+{
+    const long n3 = (long)nx * ny * nz;
+    double* b = work;                       // 2 n3: rows of 2nx
+    double* twx = b + 2 * n3;
+    double* twx2 = twx + 2 * nx;
+    double* twy = twx2 + 4 * nx;
+    double* twy2 = twy + 2 * ny;
+    const int lens[4] = {nx, 2 * nx, ny, 2 * ny};
+    double* tws[4] = {twx, twx2, twy, twy2};
+    for (int q = 0; q < 4; ++q)
+        hipLaunchKernelGGL(twiddle_kernel, dim3(nb(lens[q])), dim3(RNT), 0, s, tws[q], lens[q]);
+    if (hipGetLastError() != hipSuccess) return MHD_E_DEVICE;
+    int e = hc_pass(in, out, nx, ny, nz, 0, SM_R2HC, twx, s);                 // x forward, length nx
+    if (e) return e;
+    hipLaunchKernelGGL(pad_x_kernel, dim3(nb(2 * n3)), dim3(RNT), 0, s, out, b, nx, (long)ny * nz);
+    if ((e = hc_pass(b, b, 2 * nx, ny, nz, 0, SM_HC2R, twx2, s))) return e;   // x inverse, 2nx
+    if ((e = hc_pass(b, b, 2 * nx, ny, nz, 1, SM_R2HC, twy, s))) return e;    // y forward, ny
+    hipLaunchKernelGGL(pad_y_kernel, dim3(nb(4 * n3)), dim3(RNT), 0, s, b, out, 2 * nx, ny, nz);
+    if ((e = hc_pass(out, out, 2 * nx, 2 * ny, nz, 1, SM_HC2R, twy2, s))) return e;   // y inverse, 2ny
+    return hipGetLastError() == hipSuccess ? 0 : MHD_E_DEVICE;
 }
 
 }  // namespace
@@ -130,43 +170,35 @@ int hc_pass(double* a, int nx, int ny, int nz, int dir, int mode, const double* 
 
 using namespace mhd;
 
+
 extern "C" {
 
 int mhd_expand(int nx, int ny, int nz, const double* in, double* out)
 {
-    if (!in || !out || nz < 1) return MHD_E_ARG;
-    if (nx % 2) return MHD_E_ODD_NX;
-    if (ny % 2) return MHD_E_ODD_NY;
-    if (!spectral_len_supported(nx) || !spectral_len_supported(2 * nx) || !spectral_len_supported(ny) ||
-        !spectral_len_supported(2 * ny))
-        return MHD_E_SIZE;
+    {
+        const int e = expand_check(nx, ny, nz, in, out);
+        if (e) return e;
+    }
     const long n3 = (long)nx * ny * nz;
-    double *a = nullptr, *b = nullptr, *d = nullptr;
-    double *twx = twiddles64(nx), *twx2 = twiddles64(2 * nx), *twy = twiddles64(ny), *twy2 = twiddles64(2 * ny);
+    double *a = nullptr, *d = nullptr, *work = nullptr;
     hipStream_t s = nullptr;
     int rc = MHD_E_ALLOC;
-    if (twx && twx2 && twy && twy2 && hipMalloc((void**)&a, n3 * sizeof(double)) == hipSuccess &&
-        hipMalloc((void**)&b, 2 * n3 * sizeof(double)) == hipSuccess &&
+    if (hipMalloc((void**)&a, n3 * sizeof(double)) == hipSuccess &&
         hipMalloc((void**)&d, 4 * n3 * sizeof(double)) == hipSuccess &&
+        hipMalloc((void**)&work, expand_work_doubles(nx, ny, nz) * sizeof(double)) == hipSuccess &&
         hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess) {
         rc = MHD_E_DEVICE;
         do {
             if (hipMemcpyAsync(a, in, n3 * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) break;
-            int e = hc_pass(a, nx, ny, nz, 0, SM_R2HC, twx, s);                      // x forward, length nx
+            const int e = expand_device(nx, ny, nz, a, d, work, s);
             if (e) { rc = e; break; }
-            hipLaunchKernelGGL(pad_x_kernel, dim3(nb(2 * n3)), dim3(RNT), 0, s, a, b, nx, (long)ny * nz);
-            if ((e = hc_pass(b, 2 * nx, ny, nz, 0, SM_HC2R, twx2, s))) { rc = e; break; }   // x inverse, 2nx
-            if ((e = hc_pass(b, 2 * nx, ny, nz, 1, SM_R2HC, twy, s))) { rc = e; break; }    // y forward, ny
-            hipLaunchKernelGGL(pad_y_kernel, dim3(nb(4 * n3)), dim3(RNT), 0, s, b, d, 2 * nx, ny, nz);
-            if ((e = hc_pass(d, 2 * nx, 2 * ny, nz, 1, SM_HC2R, twy2, s))) { rc = e; break; }  // y inverse
-            if (hipGetLastError() != hipSuccess) break;
             if (hipMemcpyAsync(out, d, 4 * n3 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) break;
             if (hipStreamSynchronize(s) != hipSuccess) break;
             rc = MHD_OK;
         } while (0);
     }
     if (s) (void)hipStreamSynchronize(s);
-    for (double* p : {a, b, d, twx, twx2, twy, twy2})
+    for (double* p : {a, d, work})
         if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
     return rc;

@@ -15,7 +15,7 @@ int launch_spectral_g5(int n, const SpecLaunch& L, int f32, hipStream_t s);
 
 #define MHD_ALL_SIZES(X) X(8) X(12) X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(60) X(64) X(80) X(96) X(120) X(128) X(160) X(192) X(240) X(256) X(320) X(384) X(480) X(512) X(640) X(768) X(1024)
 
-bool spectral_len_supported(int n)
+bool spectral_len_fixed(int n)
 {
     switch (n) {
 #define X(n_) case n_: return true;
@@ -23,6 +23,13 @@ bool spectral_len_supported(int n)
 #undef X
     }
     return false;
+}
+
+bool spectral_len_supported(int n)
+{
+    if (spectral_len_fixed(n)) return true;
+    int r[SPEC_GEN_MAX_PASSES], np = 0;
+    return generic_plan(n, r, &np);
 }
 
 bool spectral_has_variant(int n)
@@ -38,6 +45,7 @@ bool spectral_has_variant(int n)
 static int launch_group(const SpecLaunch& L, int compute_f32, hipStream_t s)
 {
     const int n = (L.dir == 0) ? L.g.nx : L.g.ny;
+    if (!spectral_len_fixed(n)) return launch_spectral_generic(L, compute_f32, s);
     if (n <= 60) return launch_spectral_g0(n, L, compute_f32, s);
     if (n <= 128) return launch_spectral_g1(n, L, compute_f32, s);
     if (n <= 256) return launch_spectral_g2(n, L, compute_f32, s);

@@ -23,6 +23,9 @@ enum MulKind { MUL_D1 = 0, MUL_D2 = 1, MUL_MASK = 2, MUL_D2C = 3 };
 enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3, LD_MAXW = 4 };
 
 constexpr int SPEC_MAX_JOBS = 8;
+// lengths without a register-blocked plan: run-time radix plans up to this length
+constexpr int SPEC_GEN_MAX_N = 2048;
+constexpr int SPEC_GEN_MAX_PASSES = 12;
 
 // One transform job: one input field (or a fused product of fields) and up
 // to two outputs.  Element types: *_f32 = 1 for float arrays, 0 for double.
@@ -59,12 +62,24 @@ struct SpecLaunch {
     int variant;      // radix plan: 0 the default, 1 the length's alternative (if any)
     int max_groups;   // optional further cap on the grid; 0 = none
     int group_major;  // set by the launcher: items ordered line group first (see spec_kernel)
+    // set by the launcher for lengths without a register-blocked plan: the
+    // run-time radices and the complex lines per workgroup
+    int gen_radix[SPEC_GEN_MAX_PASSES];
+    int gen_npass;
+    int gen_nlb;
 };
 
 // Returns 0 or a negative code (-20: unsupported length).
 int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s);
 
+// every even length from 2 to SPEC_GEN_MAX_N: register-blocked kernels for
+// the lengths of fft_reg.hpp, run-time radix kernels for the others
 bool spectral_len_supported(int n);
+// the length has a register-blocked plan
+bool spectral_len_fixed(int n);
+// run-time radix plan of an even length (false: odd or too long)
+bool generic_plan(int n, int radix[SPEC_GEN_MAX_PASSES], int* npass);
+int launch_spectral_generic(const SpecLaunch& L, int compute_f32, hipStream_t s);
 // the length has an alternative radix plan (SpecLaunch::variant = 1)
 bool spectral_has_variant(int n);
 

@@ -23,6 +23,7 @@
 #include <type_traits>
 
 #include "fft_reg.hpp"
+#include "launch.hpp"
 #include "spectral.hpp"
 
 namespace mhd {
@@ -229,15 +230,19 @@ struct Lines {
 // back, so jobs that load the same fields (the cross products of u and B)
 // find the lines in their CU's caches instead of re-reading HBM.
 template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
-__global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
+struct SpecK {
+    using Args = SpecLaunch;
+    static constexpr int NT = B * plan_tpt<PL>();
+    static __device__ __forceinline__ void body(const SpecLaunch& L, uint3 bid, uint3 gdim);
+};
+
+template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
+__device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body(const SpecLaunch& L, uint3 bid, uint3 gdim)
 {
-    (void)Lv;
     constexpr int N = PL::N;
     using F = RegFFT<TC, PL>;
     using LN = Lines<PL, DIR, B>;
-    constexpr int NT = B * plan_tpt<PL>();
     constexpr int NB0 = F::NB0, NBL = F::NBL, R0 = F::R0, RL = F::RL;
-    const SpecLaunch& L = kernarg<SpecLaunch>();
     const Geom g = L.g;
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<TC>* twl = reinterpret_cast<cpx<TC>*>(smem);
@@ -247,7 +252,7 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
     LN::lane(threadIdx.x, t, j);
     cpx<TC>* lds = twl + N + t * F::SP;
     const long ngrp = LN::groups(g);
-    const long G = gridDim.x;
+    const long G = gdim.x;
     const bool gmaj = L.group_major != 0;
     const long nitems = gmaj ? ((ngrp + G - 1) / G) * G * L.njobs : ngrp * L.njobs;
     const double cx = L.cx;
@@ -315,13 +320,13 @@ __global__ void __launch_bounds__(B * plan_tpt<PL>()) spec_kernel(SpecLaunch Lv)
     };
 
     cpx<TC> nxt[R0];
-    long item = blockIdx.x;
+    long item = bid.x;
     if (item < nitems) fetch(item, nxt);
-    for (; item < nitems; item += gridDim.x) {
+    for (; item < nitems; item += G) {
         cpx<TC> v[F::RMAX];
 #pragma unroll
         for (int r = 0; r < R0; ++r) v[r] = nxt[r];
-        if (item + gridDim.x < nitems) fetch(item + gridDim.x, nxt);
+        if (item + G < nitems) fetch(item + G, nxt);
 
         int jb;
         long grp;
@@ -461,10 +466,11 @@ constexpr size_t spec_lds_bytes()
 template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
 static int launch_one(const SpecLaunch& L, hipStream_t s)
 {
-    constexpr int NT = B * plan_tpt<PL>();
+    using K = SpecK<TC, PL, DIR, B, MODE, LOP>;
+    constexpr int NT = K::NT;
     constexpr size_t lds = spec_lds_bytes<TC, PL, B>();
     static_assert(lds <= (size_t)SPEC_LDS_Y, "transform workgroup exceeds 160 KB of LDS");
-    auto kern = spec_kernel<TC, PL, DIR, B, MODE, LOP>;
+    auto kern = single_kernel<K>;
     const long items = Lines<PL, DIR, B>::groups(L.g) * L.njobs;
     // more than 64 KB of dynamic LDS is opted into once per kernel
     static int lds_ok = lds <= 65536 ? 1 : 0;
@@ -490,8 +496,7 @@ static int launch_one(const SpecLaunch& L, hipStream_t s)
     // FP32, the job-major order measured 5-10 % faster
     SpecLaunch La = L;
     La.group_major = grid < items && L.njobs > 1 && LOP == LD_CROSS && sizeof(TC) == 8;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), lds, s, La);
-    return hipGetLastError() == hipSuccess ? 0 : -22;
+    return mhd_launch<K>(dim3((unsigned)grid), dim3(NT), lds, s, La);
 }
 
 // (mode, load op) pairs the solver uses; every job of L has the same load op

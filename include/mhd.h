@@ -170,7 +170,8 @@ int mhd_kernel_timing(mhd_ctx_t* ctx, int kernel, int enable);
 int mhd_kernel_time(mhd_ctx_t* ctx, double* total_ms, long* launches);
 
 /* Dump / restart files in the reference's layout (CreateNetCDF, WriteNetCDF,
- * ReadNetCDF: src/netcdf.F90:39-1287): netCDF classic (CDF-1), dimensions
+ * ReadNetCDF: src/netcdf.F90:39-1287): netCDF classic (CDF-1; CDF-2, the
+ * 64-bit-offset variant, when a variable would begin past 2 GiB), dimensions
  * nx, ny, nz, one; global attributes version, history, gamma .. ymax;
  * variables time, iteration, timestep and R, T, U, V, W, Bx, By, Bz over
  * (nz, ny, nx), double (iprec 8) or float (iprec 4).  Host-only, no device
@@ -188,6 +189,12 @@ typedef struct mhd_dump {
 } mhd_dump_t;
 int mhd_write_dump(const char* path, const mhd_dump_t* meta, const double* const fields[8], int iprec);
 int mhd_read_dump(const char* path, mhd_dump_t* meta, double* const fields[8]);
+/* The header mhd_write_dump writes for this shape and precision (the bytes
+ * before the time variable's value): returns its length, copied to buf when
+ * cap is large enough, and the format version (1 or 2) in *version; or
+ * MHD_E_ARG (a field of 4 GiB or more, which neither format holds).  No
+ * counterpart in the reference, whose nf_create lays the header out. */
+long mhd_dump_header(const mhd_dump_t* meta, int iprec, unsigned char* buf, long cap, int* version);
 
 /* ---- autotuner (no counterpart in the reference) ---------------------------
  * Times this box's launch shapes on this GPU and keeps the fastest in the
@@ -207,18 +214,22 @@ int mhd_set_tune(mhd_ctx_t* c, const mhd_tune_t* t);
 /* ---- ensembles of independent boxes (no counterpart in the reference) ------
  * The boxes (contexts with a state, any sizes, parameters and precisions on
  * this GPU) are brought to steady AB3 coefficients at the fixed step dt (up
- * to two start-up steps each), then the kernels of MHD_ENSEMBLE_BLOCK steps
- * of every box are captured into one hipGraph, each box on its own stream
- * branch.  mhd_ensemble_step launches it nsteps / block times (nsteps a
- * multiple of the block) and advances each box's driver scalars; results
- * equal mhd_step(box, nsteps, dt, 0, NULL) per box.  Diagnostics are taken
- * per box between calls.  A box must not be stepped on its own while in an
- * ensemble. */
+ * to two start-up steps each).  The kernel launches of MHD_ENSEMBLE_BLOCK
+ * steps of every box are then recorded, and the k-th launches of all boxes
+ * of a class (boxes with the same horizontal lengths, precision and options
+ * launch the same kernels) become one launch over a table of the boxes'
+ * arguments and grids; these batched launches, one graph branch per class,
+ * are captured into one hipGraph.  mhd_ensemble_step launches it
+ * nsteps / block times (nsteps a multiple of the block) and advances each
+ * box's driver scalars; results equal mhd_step(box, nsteps, dt, 0, NULL) per
+ * box, bit for bit.  Diagnostics are taken per box between calls.  A box
+ * must not be stepped on its own while in an ensemble. */
 typedef struct mhd_ensemble mhd_ensemble_t;
 int mhd_ensemble_create(mhd_ctx_t* const* boxes, int nbox, double dt, mhd_ensemble_t** out);
 int mhd_ensemble_step(mhd_ensemble_t* e, int nsteps);
 int mhd_ensemble_block(void);
-long mhd_ensemble_nodes(mhd_ensemble_t* e);   /* graph nodes (kernels and joins) */
+long mhd_ensemble_nodes(mhd_ensemble_t* e);   /* graph nodes (batched kernels and joins) */
+int mhd_ensemble_classes(mhd_ensemble_t* e);  /* classes of boxes (graph branches) */
 void mhd_ensemble_destroy(mhd_ensemble_t* e);
 
 /* ---- resampling and coarse-graining (host arrays in, host arrays out) ---- */
@@ -239,8 +250,13 @@ int mhd_box_filter(int nx, int ny, int nz, int f, const double* in, double* out)
  * A stencil MLP maps the 8 coarse fields (r t u v w bx by bz) on a cell's
  * 3x3x3 neighbourhood (216 inputs; x, y periodic, z clamped) through two
  * 64-unit GELU layers to 12 subgrid fluxes: tau_xx yy zz xy xz yz, eps_x y z,
- * q_x y z.  Its targets are box filters (f x f x f block means, the
- * coarse-graining of interpolate/main.F90:299-306) of fine states:
+ * q_x y z.  Its targets are filters F of fine states (nx, ny, nz) onto the
+ * coarse grid (nx/f, ny/f, cz = nz/f): the f x f block mean horizontally
+ * (the coarse-graining of interpolate/main.F90:299-306) and, in z, centred on
+ * the coarse grid's own planes Z_K = K/(cz-1): the mean over
+ * [Z_K - H/2, Z_K + H/2] (H = 1/(cz-1)) of the fine profile's linear
+ * interpolant, and the fine wall planes themselves at K = 0 and cz-1, so
+ * filtered fields keep the wall values:
  *   tau_ij = F(r) [F(u_i u_j) - F(u_i) F(u_j)]
  *            - fmag ([F(B_i B_j) - F(B_i) F(B_j)] - delta_ij (trace) / 2)
  *   eps = F(u x B) - F(u) x F(B),   q_i = F(u_i T) - F(u_i) F(T).
@@ -262,6 +278,11 @@ int mhd_closure_targets(int nx, int ny, int nz, int f, double fmag, const double
 /* append one coarse snapshot (all snapshots share one shape) to the data set */
 int mhd_closure_add(mhd_closure_t* c, int cx, int cy, int cz, const double* const coarse[8],
                     const double* const target[12]);
+/* the same for the current state of a fine run: the box-filtered fields and
+ * targets (mhd_closure_targets' filter, factor f, fmag = the run's F) are
+ * computed from the run's device arrays straight into the data set, with no
+ * host round trip of the fields */
+int mhd_closure_add_state(mhd_closure_t* c, mhd_ctx_t* fine, int f);
 int mhd_closure_samples(mhd_closure_t* c, long* n);
 /* the scaling fitted to the data set (until set explicitly) */
 int mhd_closure_get_norm(mhd_closure_t* c, double norm[40]);

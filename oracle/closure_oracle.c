@@ -8,7 +8,9 @@
  * restated in orc_box_filter) and its field order (src/netcdf.F90: R, T, U, V,
  * W, Bx, By, Bz).
  *
- * Definitions (F = the f x f x f block mean, fields r t u v w bx by bz):
+ * Definitions (F = the filter of orc_closure_targets: the f x f block mean
+ * horizontally, a window centred on each coarse plane in z; fields r t u v w
+ * bx by bz):
  *   R_ij = F(u_i u_j) - F(u_i) F(u_j),  M_ij = F(B_i B_j) - F(B_i) F(B_j)
  *   tau_ij = F(r) R_ij - fmag (M_ij - delta_ij (M_xx + M_yy + M_zz) / 2)
  *            order xx yy zz xy xz yz                         (targets 0..5)
@@ -71,35 +73,78 @@ static double dgelu(double z)
     return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * (GK + 3.0 * GKA * z * z);
 }
 
+/* The filter of the closure data: the f x f block mean horizontally; in z,
+ * centred on the coarse planes Z_K = K / (cz - 1): the fine wall planes at
+ * K = 0 and cz - 1, elsewhere the mean over [Z_K - H/2, Z_K + H/2]
+ * (H = 1 / (cz - 1)) of the linear interpolant of the fine profile (fine
+ * planes at z_k = k / (nz - 1)).  orc_zweight returns the weight of fine
+ * plane k in coarse plane K, integrating the interpolant's hat function of
+ * plane k over the window, normalised so a plane's weights sum to 1. */
+static double zwin_raw(int nz, int cz, int K, int k)
+{
+    const double Z = (double)K / (double)(cz - 1), half = 0.5 / (double)(cz - 1);
+    const double a = Z - half, b = Z + half;
+    double w = 0.0;
+    /* the hat of plane k spans [z_{k-1}, z_{k+1}]: its left half rises on
+     * interval k-1, its right half falls on interval k */
+    for (int side = 0; side < 2; side++) {
+        const int i = side == 0 ? k - 1 : k;          /* interval [z_i, z_{i+1}] */
+        if (i < 0 || i > nz - 2) continue;
+        const double zi = (double)i / (double)(nz - 1), zi1 = (double)(i + 1) / (double)(nz - 1);
+        const double p = a > zi ? a : zi, q = b < zi1 ? b : zi1;
+        if (!(q > p)) continue;
+        const double d = zi1 - zi, tp = (p - zi) / d, tq = (q - zi) / d;
+        /* integral over [p, q] of t (rising side) or 1 - t (falling side) */
+        const double rise = d * 0.5 * (tq * tq - tp * tp);
+        w += side == 0 ? rise : d * (tq - tp) - rise;
+    }
+    return w;
+}
+
+double orc_zweight(int nz, int cz, int K, int k)
+{
+    if (K == 0) return k == 0 ? 1.0 : 0.0;
+    if (K == cz - 1) return k == nz - 1 ? 1.0 : 0.0;
+    double sum = 0.0;
+    for (int j = 0; j < nz; j++) sum += zwin_raw(nz, cz, K, j);
+    return zwin_raw(nz, cz, K, k) / sum;
+}
+
 int orc_closure_targets(int nx, int ny, int nz, int f, double fmag, const double* const fine[8],
                         double* const coarse[8], double* const target[12])
 {
-    if (f < 1 || nx % f || ny % f || nz % f) return -1;
+    if (f < 1 || nx % f || ny % f || nz % f || nz / f < 2) return -1;
     const int cx = nx / f, cy = ny / f, cz = nz / f;
-    const double inv = 1.0 / ((double)f * f * f);
-    for (int K = 0; K < cz; K++)
+    const double inv = 1.0 / ((double)f * f);
+    double* wz = (double*)malloc(sizeof(double) * (size_t)nz);
+    if (!wz) return -5;
+    for (int K = 0; K < cz; K++) {
+        for (int k = 0; k < nz; k++) wz[k] = orc_zweight(nz, cz, K, k);
         for (int J = 0; J < cy; J++)
             for (int I = 0; I < cx; I++) {
                 double s[8] = {0}, uu[6] = {0}, bb[6] = {0}, uxb[3] = {0}, ut[3] = {0};
-                for (int kk = 0; kk < f; kk++)
+                for (int k = 0; k < nz; k++) {
+                    const double w = wz[k];
+                    if (w == 0.0) continue;
                     for (int jj = 0; jj < f; jj++)
                         for (int ii = 0; ii < f; ii++) {
-                            const size_t q = ((size_t)(f * K + kk) * ny + (f * J + jj)) * nx + (f * I + ii);
+                            const size_t q = ((size_t)k * ny + (f * J + jj)) * nx + (f * I + ii);
                             double v[8];
                             for (int n = 0; n < 8; n++) {
                                 v[n] = fine[n][q];
-                                s[n] += v[n];
+                                s[n] += w * v[n];
                             }
-                            const double u = v[2], vv = v[3], w = v[4], bx = v[5], by = v[6], bz = v[7], t = v[1];
-                            uu[0] += u * u; uu[1] += vv * vv; uu[2] += w * w;
-                            uu[3] += u * vv; uu[4] += u * w; uu[5] += vv * w;
-                            bb[0] += bx * bx; bb[1] += by * by; bb[2] += bz * bz;
-                            bb[3] += bx * by; bb[4] += bx * bz; bb[5] += by * bz;
-                            uxb[0] += vv * bz - w * by;
-                            uxb[1] += w * bx - u * bz;
-                            uxb[2] += u * by - vv * bx;
-                            ut[0] += u * t; ut[1] += vv * t; ut[2] += w * t;
+                            const double u = v[2], vv = v[3], ww = v[4], bx = v[5], by = v[6], bz = v[7], t = v[1];
+                            uu[0] += w * (u * u); uu[1] += w * (vv * vv); uu[2] += w * (ww * ww);
+                            uu[3] += w * (u * vv); uu[4] += w * (u * ww); uu[5] += w * (vv * ww);
+                            bb[0] += w * (bx * bx); bb[1] += w * (by * by); bb[2] += w * (bz * bz);
+                            bb[3] += w * (bx * by); bb[4] += w * (bx * bz); bb[5] += w * (by * bz);
+                            uxb[0] += w * (vv * bz - ww * by);
+                            uxb[1] += w * (ww * bx - u * bz);
+                            uxb[2] += w * (u * by - vv * bx);
+                            ut[0] += w * (u * t); ut[1] += w * (vv * t); ut[2] += w * (ww * t);
                         }
+                }
                 double m[8];
                 for (int n = 0; n < 8; n++) m[n] = s[n] * inv;
                 const double r = m[0], T = m[1], U[3] = {m[2], m[3], m[4]}, B[3] = {m[5], m[6], m[7]};
@@ -118,6 +163,8 @@ int orc_closure_targets(int nx, int ny, int nz, int f, double fmag, const double
                 target[8][o] = uxb[2] * inv - (U[0] * B[1] - U[1] * B[0]);
                 for (int c = 0; c < 3; c++) target[9 + c][o] = ut[c] * inv - U[c] * T;
             }
+    }
+    free(wz);
     return 0;
 }
 

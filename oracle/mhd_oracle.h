@@ -137,6 +137,7 @@ int orc_closure_nparam(void);
 int orc_closure_attach(orc_ctx* c, const double* params, const double norm[40]);
 int orc_closure_fix_fluxes(orc_ctx* c, const double* flux);
 float orc_bf16(float x);
+double orc_zweight(int nz, int cz, int K, int k);
 int orc_closure_targets(int nx, int ny, int nz, int f, double fmag, const double* const fine[8],
                         double* const coarse[8], double* const target[12]);
 int orc_mlp_forward(int nx, int ny, int nz, const double* const coarse[8], const double* params,

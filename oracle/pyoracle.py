@@ -63,6 +63,8 @@ def lib():
         L.orc_box_filter.argtypes = [ci, ci, ci, ci, dp, dp]
         pp = ctypes.POINTER(dp)
         L.orc_closure_targets.argtypes = [ci, ci, ci, ci, ctypes.c_double, pp, pp, pp]
+        L.orc_zweight.argtypes = [ci, ci, ci, ci]
+        L.orc_zweight.restype = ctypes.c_double
         L.orc_mlp_forward.argtypes = [ci, ci, ci, pp, dp, dp, ci, pp]
         L.orc_mlp_grad.argtypes = [ci, ci, ci, pp, pp, dp, dp, ci, ctypes.POINTER(ctypes.c_long), ci,
                                    dp, dp]

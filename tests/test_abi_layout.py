@@ -1,0 +1,96 @@
+"""The structs of include/mhd.h have the same size and field offsets in C,
+in the ctypes binding (fug30v0kxc_amd/_abi.py) and in the Fortran interface
+module (fug30v0kxc_amd/host/mhd_capi.f90): a host whose copy of a struct is
+shorter than the library's would have the library read past its end.
+Compiled on the CPU; no device calls."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from fug30v0kxc_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mhd.h")
+CAPI = os.path.join(ROOT, "fug30v0kxc_amd", "host", "mhd_capi.f90")
+
+STRUCTS = {
+    "mhd_params_t": (_abi.MhdParams, ["gamma", "sf", "t_amp", "b_amp", "eta_c", "eta_z0", "eta_w", "opts"]),
+    "mhd_scalars_t": (_abi.MhdScalars, ["dt", "bzbar", "nit", "second"]),
+    "mhd_diag_t": (_abi.MhdDiag, ["ubar", "divbmin", "dt", "nit"]),
+    "mhd_dump_t": (_abi.MhdDump, ["nx", "iteration", "time", "timestep", "gamma", "ymax", "history"]),
+    "mhd_tune_t": (_abi.MhdTune, ["variant_x", "zc", "ms_x", "ms_col"]),
+}
+
+
+def _c_layout(tmp_path):
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', 'int main(void) {']
+    for st, (_, fields) in STRUCTS.items():
+        lines.append(f'  printf("{st} size %zu\\n", sizeof({st}));')
+        for f in fields:
+            lines.append(f'  printf("{st} {f} %zu\\n", offsetof({st}, {f}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout_c"
+    subprocess.check_call(["gcc", "-std=c11", "-o", str(exe), str(src)])
+    return _parse(subprocess.check_output([str(exe)], text=True))
+
+
+def _parse(text):
+    out = {}
+    for ln in text.splitlines():
+        parts = ln.split()
+        if len(parts) == 3:
+            out[(parts[0], parts[1])] = int(parts[2])
+    return out
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    c = _c_layout(tmp_path)
+    for st, (cls, fields) in STRUCTS.items():
+        assert ctypes.sizeof(cls) == c[(st, "size")], st
+        for f in fields:
+            assert getattr(cls, f).offset == c[(st, f)], (st, f)
+
+
+@pytest.mark.skipif(shutil.which("amdflang") is None, reason="no Fortran compiler")
+def test_fortran_structs_match_the_header(tmp_path):
+    c = _c_layout(tmp_path)
+    decl, body = [], []
+    for st, (_, fields) in STRUCTS.items():
+        if st == "mhd_tune_t":
+            continue   # the Fortran hosts do not tune
+        v = "v_" + st
+        decl.append(f"  type({st}), target :: {v}")
+        body.append(f"  print '(a,1x,a,1x,i0)', '{st}', 'size', c_sizeof({v})")
+        for f in fields:
+            comp = f"{v}%{f}" + ("(1)" if f == "history" else "")
+            body.append(f"  print '(a,1x,a,1x,i0)', '{st}', '{f}', "
+                        f"transfer(c_loc({comp}), 0_c_intptr_t) - transfer(c_loc({v}), 0_c_intptr_t)")
+    src = tmp_path / "layout.f90"
+    src.write_text("program layout\n  use iso_c_binding\n  use mhd_capi\n  implicit none\n"
+                   + "\n".join(decl) + "\n" + "\n".join(body) + "\nend program\n")
+    exe = tmp_path / "layout_f"
+    libdir = os.path.join(ROOT, "fug30v0kxc_amd")
+    subprocess.check_call(["amdflang", "-J", str(tmp_path), "-o", str(exe), CAPI, str(src),
+                           "-L" + libdir, "-lmhd_hip", "-Wl,-rpath," + libdir], cwd=str(tmp_path))
+    f = _parse(subprocess.check_output([str(exe)], text=True))
+    for (st, name), off in f.items():
+        assert off == c[(st, name)], (st, name, off, c[(st, name)])
+    assert ("mhd_params_t", "opts") in f
+
+
+def test_integration_snippet_matches_the_module():
+    """The mhd_params_t shown in INTEGRATION.md declares the same components,
+    with the same defaults, as the interface module."""
+    import re
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    m = re.search(r"type, bind\(C\) :: mhd_params_t\n(.*?)end type", doc, re.S)
+    assert m, "INTEGRATION.md shows mhd_params_t"
+    mod = open(CAPI).read()
+    n = re.search(r"type, bind\(C\) :: mhd_params_t\n(.*?)end type", mod, re.S)
+    norm = lambda s: [ln.strip() for ln in s.strip().splitlines()]
+    assert norm(m.group(1)) == norm(n.group(1))

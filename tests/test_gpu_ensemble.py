@@ -61,3 +61,41 @@ def test_ensemble_rejects_bad_requests():
         Ensemble([bs[0], bs[0]], DT)
     for b in bs:
         b.close()
+
+
+def test_ensemble_batches_each_class_into_one_launch_per_kernel():
+    """Boxes with the same horizontal lengths and precision form one class
+    whatever their nz, Ra, Q or eta(z): their launches are batched, so the
+    graph holds one node per kernel of a class, not per box."""
+    from fug30v0kxc_amd import MHD_PREC_MIXED, Ensemble, Solver
+    print("[ensemble] classes", flush=True)
+    specs = [((32, 32, 12), reference_params(ra=1e5, q=100.0, t_amp=1e-3, b_amp=1e-3)),
+             ((32, 32, 20), dict(reference_params(ra=3e5, q=30.0, t_amp=1e-3, b_amp=1e-3),
+                                 eta_c=20.0, eta_z0=0.5, eta_w=0.05)),
+             ((32, 32, 16), reference_params(ra=1e4, q=500.0, t_amp=1e-3, b_amp=1e-3)),
+             ((48, 32, 12), reference_params(ra=1e5, q=100.0, t_amp=1e-3, b_amp=1e-3))]
+
+    def make():
+        out = []
+        for shape, p in specs:
+            s = Solver(*shape, p, precision=MHD_PREC_MIXED)
+            s.static()
+            out.append(s)
+        return out
+    ens_boxes, ref_boxes = make(), make()
+    with Ensemble(ens_boxes, DT) as ens:
+        assert ens.classes == 2
+        one = Ensemble([ref_boxes[0]], DT)
+        per_class = one.nodes
+        one.close()
+        # two classes: about twice one box's nodes, not four times
+        assert ens.nodes < 2 * per_class + 16
+        ens.step(60)
+    for b in ref_boxes:
+        b.step(60 if b is ref_boxes[0] else 62, DT, 0)
+    for i, (e, r) in enumerate(zip(ens_boxes, ref_boxes)):
+        se, sr = e.get_state(), r.get_state()
+        for n in NAMES:
+            assert np.array_equal(se[n], sr[n]), (i, n)
+    for b in ens_boxes + ref_boxes:
+        b.close()

@@ -147,3 +147,62 @@ def test_interpolation_tool(tmp_path):
         want = pyoracle.expand(fields[n])
         assert got[n].shape == (6, 24, 32)
         assert np.abs(got[n] - want).max() <= 1e-12 * np.abs(want).max(), n
+
+
+# ---- the reference's own smoke inputs ----------------------------------------
+# bin/originalINPUT (4 x 4 x 8, gamma 5/3) and bin/test1/INPUT (30 x 30 x 60,
+# gamma 1.4), copied unchanged to tests/golden/reference_inputs: Ra 1e5,
+# Q 2500, a static start with the reference's T noise (0.1), calcdt every 10
+# steps, 10 steps with diagnostics every 2.  No INPUT.ext: every setting is
+# the reference's.  They write no dump (NDU16 210 > TOTAL), so the printed
+# diagnostics are compared with the oracle's at each NDU06 step.
+REF_INPUTS = os.path.join(ROOT, "tests", "golden", "reference_inputs")
+
+
+def _diag_lines(out):
+    """Per NDU06 step: {key: value} from the host's printout."""
+    steps, cur = [], None
+    for ln in out.splitlines():
+        f = ln.replace("=", " ").split()
+        if not f:
+            continue
+        if f[0] == "NIT":
+            cur = {"nit": int(f[1]), "ttime": float(f[3])}
+            steps.append(cur)
+        elif cur is not None and f[0] in ("Nusselt", "kinenergy", "magenergy", "dt"):
+            cur[f[0].lower()] = float(f[1])
+        elif cur is not None and f[0] in ("rmin", "tmin", "wmin", "bzmin"):
+            cur[f[0]] = float(f[1])
+            cur[f[2]] = float(f[3])
+    return steps
+
+
+@pytest.mark.parametrize("name,box", [("originalINPUT", (4, 4, 8)), ("test1_INPUT", (30, 30, 60))])
+def test_reference_smoke_inputs(tmp_path, name, box):
+    if not os.path.exists(EXE):
+        pytest.skip("p3d_gpu not built")
+    text = open(os.path.join(REF_INPUTS, name)).read()
+    (tmp_path / "INPUT").write_text(text)
+    print(f"[host] p3d_gpu on the reference's {name}", flush=True)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got = _diag_lines(r.stdout)
+    assert [d["nit"] for d in got] == [2, 4, 6, 8, 10]
+    vals = text.split("\n")
+    gamma = float(vals[5].split()[0])
+    p = reference_params(ra=1e5, q=2500.0, gamma=gamma, t_amp=0.1)
+    o = Oracle(*box, p)
+    o.setup_static()
+    prev_dt = None
+    for d in got:
+        od = o.step(2, 0.0, 2)
+        assert abs(d["ttime"] - o.scalar("ttime")) <= 1e-12 * o.scalar("ttime")
+        # the printed dt is the step's; calcdt at every 10th step comes after it
+        dt_used = o.scalar("dt") if d["nit"] % 10 else prev_dt
+        prev_dt = o.scalar("dt")
+        assert abs(d["dt"] - dt_used) <= 1e-10 * dt_used
+        for key, i in (("kinenergy", 2), ("magenergy", 3), ("nusselt", 4)):
+            assert abs(d[key] - od[i]) <= 1e-10 * abs(od[i]), (d["nit"], key, d[key], od[i])
+        for n, lo, hi in (("r", "rmin", "rmax"), ("t", "tmin", "tmax"), ("w", "wmin", "wmax")):
+            scale = np.abs(o[n]).max()
+            assert abs(d[lo] - o[n].min()) <= 1e-10 * scale and abs(d[hi] - o[n].max()) <= 1e-10 * scale

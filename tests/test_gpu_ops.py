@@ -10,7 +10,10 @@ from tests.conftest import reference_params
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(16, 12, 8), (64, 64, 48), (30, 30, 12), (96, 64, 10), (256, 256, 6), (512, 512, 6)]
+SIZES = [(16, 12, 8), (64, 64, 48), (30, 30, 12), (96, 64, 10), (256, 256, 6), (512, 512, 6),
+         # lengths without a register-blocked plan (run-time radices, prime
+         # factors up to 101), including the reference's 4 x 4 x 8 smoke box
+         (4, 4, 8), (2, 8, 6), (10, 14, 6), (36, 22, 7), (100, 50, 6), (202, 34, 6), (2048, 6, 6)]
 
 
 def rel(a, b):
@@ -103,7 +106,7 @@ def test_unknown_flag_is_code_7(pair):
 def test_create_error_codes():
     from fug30v0kxc_amd import MhdError, Solver
     p = reference_params()
-    for dims, code in (((15, 16, 8), -1), ((16, 15, 8), -2), ((16, 16, 5), 1)):
+    for dims, code in (((15, 16, 8), -1), ((16, 15, 8), -2), ((16, 16, 5), 1), ((2050, 16, 8), -20)):
         with pytest.raises(MhdError) as e:
             Solver(*dims, p)
         assert e.value.code == code

@@ -29,19 +29,21 @@ def test_bf16_round_to_nearest_even():
 
 
 def test_targets_vanish_for_block_constant_fields():
+    # constant over each f x f column block and uniform in z: nothing is left
+    # below the filter
     f = 2
     rng = np.random.default_rng(1)
-    coarse = {n: rng.standard_normal((3, 4, 4)) + 3.0 for n in F}
-    fine = {n: np.kron(coarse[n], np.ones((f, f, f))) for n in F}
+    plane = {n: rng.standard_normal((4, 4)) + 3.0 for n in F}
+    fine = {n: np.broadcast_to(np.kron(plane[n], np.ones((f, f)))[None], (6, 8, 8)).copy() for n in F}
     c, t = po.closure_targets(fine, f, fmag=0.7)
     assert np.abs(t).max() < 1e-12
     for n in F:
-        assert np.allclose(c[n], coarse[n], rtol=0, atol=1e-13)
+        assert np.allclose(c[n], np.broadcast_to(plane[n], (3, 4, 4)), rtol=0, atol=1e-13)
 
 
 @pytest.mark.parametrize("f", [2, 4])
 def test_targets_closed_forms_for_linear_fields(f):
-    nz, ny, nx = f, f, 2 * f
+    nz, ny, nx = 2 * f, f, 2 * f     # two coarse planes: the walls, no z averaging
     shape = (nz, ny, nx)
     i = np.broadcast_to(np.arange(nx, dtype=float), shape).copy()   # fine x index
     var = (f * f - 1) / 12.0                                        # variance of f consecutive integers
@@ -73,11 +75,46 @@ def test_targets_closed_forms_for_linear_fields(f):
     assert np.allclose(t[9], var) and np.abs(t[10:]).max() < 1e-12
 
 
-def test_coarse_fields_equal_box_filter():
-    fine = _fine((8, 8, 12), seed=2)
-    c, _ = po.closure_targets(fine, 4, 1.0)
+def test_coarse_fields_horizontal_block_mean_and_z_vertices():
+    # horizontally the coarse fields are the f x f block means of each fine
+    # plane; in z the wall planes are the fine wall planes, and a linear
+    # profile a + b z is reproduced exactly at the coarse planes K / (cz - 1)
+    nz, ny, nx, f = 24, 8, 12, 4
+    cz = nz // f
+    fine = _fine((nz, ny, nx), seed=2)
+    c, _ = po.closure_targets(fine, f, 1.0)
     for n in F:
-        assert np.array_equal(c[n], po.box_filter(fine[n], 4))
+        for k, K in ((0, 0), (nz - 1, cz - 1)):
+            want = fine[n][k].reshape(ny // f, f, nx // f, f).mean(axis=(1, 3))
+            assert np.allclose(c[n][K], want, rtol=0, atol=1e-13)
+    z = np.arange(nz) / (nz - 1.0)
+    Z = np.arange(cz) / (cz - 1.0)
+    x = np.arange(nx) / nx
+    lin = {n: (1.5 + 0.7 * z)[:, None, None] + 0 * x[None, None, :] + np.zeros((nz, ny, nx)) for n in F}
+    c, _ = po.closure_targets(lin, f, 1.0)
+    assert np.abs(c["t"] - (1.5 + 0.7 * Z)[:, None, None]).max() < 1e-14
+
+
+@pytest.mark.parametrize("nz,cz", [(24, 6), (384, 96), (40, 7), (13, 13)])
+def test_z_weights_are_a_centred_window(nz, cz):
+    """Each coarse plane's weights sum to 1; an interior plane's weights
+    centre on Z_K (first moment) and give the window mean of the fine
+    linear interpolant of z^2 (second moment, against dense sampling)."""
+    z = np.arange(nz) / (nz - 1.0)
+    H = 1.0 / (cz - 1)
+    for K in range(cz):
+        w = np.array([po.lib().orc_zweight(nz, cz, K, k) for k in range(nz)])
+        assert abs(w.sum() - 1.0) < 1e-14
+        if K in (0, cz - 1):
+            assert w[0 if K == 0 else -1] == 1.0
+            continue
+        Zk = K * H
+        assert abs((w * z).sum() - Zk) < 1e-14
+        # the interpolant of z^2 integrated over the window, by fine sub-sampling
+        a, b = Zk - H / 2, Zk + H / 2
+        zz = np.linspace(a, b, 200001)
+        want = np.trapezoid(np.interp(zz, z, z * z), zz) / H
+        assert abs((w * z * z).sum() - want) < 1e-9
 
 
 def _norm(seed=3):
