@@ -21,6 +21,8 @@ MHD_E_ARG = -10
 MHD_E_SIZE = -20
 MHD_E_DEVICE = -30
 MHD_E_STATE = -40
+MHD_E_SCRATCH = -50
+MHD_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # MHD_OPT_* switches of mhd_params_t.opts
 OPT_RADBC, OPT_CONSTFLUX, OPT_PERFECT, OPT_KINEMATIC, OPT_DYNAMO = 1, 2, 4, 8, 16
@@ -134,6 +136,7 @@ def load():
         "mhd_ensemble_classes": (i, [vp]),
         "mhd_ensemble_destroy": (None, [vp]),
         "mhd_expand": (i, [i, i, i, dp, dp]),
+        "mhd_expand_noalloc": (i, [i, i, i, vp, vp, vp]),
         "mhd_dump_header": (ctypes.c_long, [ctypes.POINTER(MhdDump), i, ctypes.c_char_p, ctypes.c_long,
                                             ctypes.POINTER(i)]),
         "mhd_box_filter": (i, [i, i, i, i, dp, dp]),

@@ -969,6 +969,7 @@ const char* mhd_strerror(int code)
     case MHD_E_SIZE: return "horizontal size has no transform kernel";
     case MHD_E_DEVICE: return "HIP runtime error";
     case MHD_E_STATE: return "call out of order";
+    case MHD_E_SCRATCH: return "box too large for the no-allocation scratch";
     }
     return "unknown error";
 }

@@ -12,6 +12,7 @@
 // a factor 2 against the transform's 1/(2n)).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/mhd.h"
 #include "physics.hpp"
@@ -170,6 +171,33 @@ int expand_device(int nx, int ny, int nz, const double* in, double* out, double*
 
 using namespace mhd;
 
+// Work memory of mhd_expand_noalloc: a fixed device array.  Every GPU the
+// library runs on holds its own copy, at its own address, until the process
+// ends.  Host code never uses the array's name as a device address: it takes
+// the address hipGetSymbolAddress returns for the current device.
+static constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)MHD_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES];
+
+// the device address of `scratch` on the current device, looked up once per device
+int scratch_on_device(double** ptr)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES) return MHD_E_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MHD_E_DEVICE;
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
 
 extern "C" {
 
@@ -202,6 +230,21 @@ int mhd_expand(int nx, int ny, int nz, const double* in, double* out)
         if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
     return rc;
+}
+
+int mhd_expand_noalloc(int nx, int ny, int nz, const double* in, double* out, void* stream)
+{
+    {
+        const int e = expand_check(nx, ny, nz, in, out);
+        if (e) return e;
+    }
+    if (expand_work_doubles(nx, ny, nz) > NOALLOC_SCRATCH_DOUBLES) return MHD_E_SCRATCH;
+    double* work = nullptr;
+    {
+        const int e = scratch_on_device(&work);
+        if (e) return e;
+    }
+    return expand_device(nx, ny, nz, in, out, work, (hipStream_t)stream);
 }
 
 int mhd_box_filter(int nx, int ny, int nz, int f, const double* in, double* out)

@@ -39,7 +39,8 @@ enum {
     MHD_E_ARG = -10,         /* null or inconsistent argument */
     MHD_E_SIZE = -20,        /* horizontal size has no transform kernel */
     MHD_E_DEVICE = -30,      /* HIP runtime error */
-    MHD_E_STATE = -40        /* call out of order (e.g. step before state) */
+    MHD_E_STATE = -40,       /* call out of order (e.g. step before state) */
+    MHD_E_SCRATCH = -50      /* box too large for a no-allocation entry point */
 };
 
 enum { MHD_PREC_FP64 = 0, MHD_PREC_MIXED = 1 };
@@ -239,6 +240,18 @@ void mhd_ensemble_destroy(mhd_ensemble_t* e);
  * kept as a cosine of the fine grid.  Codes: -1/-2 odd nx/ny, -20 a length of
  * nx, 2nx, ny, 2ny without a transform kernel. */
 int mhd_expand(int nx, int ny, int nz, const double* in, double* out);
+/* mhd_expand on device arrays, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  in
+ * (nx,ny,nz) and out (2nx,2ny,nz) are device pointers of the current
+ * device; in is only read.  The work is queued on `stream` (a hipStream_t;
+ * NULL = the null stream) and is complete when the stream is.  Work memory
+ * is a fixed device array of MHD_NOALLOC_SCRATCH_BYTES per GPU, which must
+ * hold 2 nx ny nz + 6 (nx + ny) doubles: larger boxes return MHD_E_SCRATCH.
+ * Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (other streams, other threads) race.  Results equal mhd_expand's. */
+enum { MHD_NOALLOC_SCRATCH_BYTES = 64 << 20 };
+int mhd_expand_noalloc(int nx, int ny, int nz, const double* in, double* out, void* stream);
 /* Block box filter of factor f (the closure data path: filtered fine fields
  * as coarse targets): out(I,J,K) = mean of in over the f x f x f block at
  * (fI, fJ, fK); (nx,ny,nz) -> (nx/f,ny/f,nz/f).  No counterpart in the
