@@ -201,7 +201,7 @@ def ensemble_bench(steps, seed=C4_SEED):
             "capture_s": round(t_capture, 2), "identical_to_sequential": bool(same)}
 
 
-def closure_bench(nfine, f=4, ra=1e6, seed=20261203):
+def closure_bench(nfine, f=4, ra=1e6, nblocks=4, seed=20261203):
     """Learned closure: a fine run (nfine^2 x 3 nfine / 4, Ra `ra`, Q 500, T
     and Bz noise 0.1 and 0.01, mixed precision) is coarse-grained (factor f
     horizontally, onto the coarse vertices in z) into snapshots straight from
@@ -260,15 +260,15 @@ def closure_bench(nfine, f=4, ra=1e6, seed=20261203):
         probe.set_state(*[coarse_state[n] for n in ("r", "t", "u", "v", "w", "bx", "by", "bz")])
         return probe.nusselt()
 
-    # a posteriori: 4 blocks from the held-out state
+    # a posteriori: nblocks blocks from the held-out state
     ref = []
     t2 = time.perf_counter()
-    for _ in range(4):
+    for _ in range(nblocks):
         d = fine.step(blk, dtf, blk)
         fine.sync()
         ref.append((closure_targets(fine.get_state(), f, p["f"])[0], d["nusselt"]))
-        note(f"reference block {len(ref)} of 4")
-    fine_ms_per_step = (time.perf_counter() - t2) / (4 * blk) * 1e3
+        note(f"reference block {len(ref)} of {nblocks}")
+    fine_ms_per_step = (time.perf_counter() - t2) / (nblocks * blk) * 1e3
     fine.close()
     nu_ref = [nu_filtered(st) for st, _ in ref]
     probe.close()
@@ -494,7 +494,7 @@ def roofline_of(r, kernel, elem, workload="c2", second=False):
     kms, nl = (r["kms2"], r["klaunch2"]) if second else (r["kms"], r["klaunch"])
     avg_ms = kms / max(nl, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = measured_traffic(kernel, "fp64" if elem == 8 else "mixed") if workload == "c2" else None
+    traffic = measured_traffic("fp64" if elem == 8 else "mixed", kernel) if workload == "c2" else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic, "kernel": KERNEL_DESC[kernel],
@@ -517,6 +517,7 @@ def main():
                     help="c5 (closure) workload: fine nx = ny (nz = 3/4 of it); 512 is the 512^2 x 384 case")
     ap.add_argument("--c5-factor", type=int, default=4, help="c5: coarse-graining factor")
     ap.add_argument("--c5-ra", type=float, default=1e6, help="c5: Rayleigh number")
+    ap.add_argument("--c5-blocks", type=int, default=4, help="c5: a-posteriori blocks of 500 fine steps")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4", "c5", "aux"],
                     help="c2: 256x256x160 (default); c3: 512x512x256 with the eta(z) step; "
                          "c4: ensemble of 32 boxes in hipGraph replays; c5: learned closure; "
@@ -571,7 +572,7 @@ def main():
         print(json.dumps(out))
         return 0
     if args.workload == "c5":
-        r = closure_bench(args.c5_fine, args.c5_factor, args.c5_ra)
+        r = closure_bench(args.c5_fine, args.c5_factor, args.c5_ra, args.c5_blocks)
         out = {"metric": "closure: explained subgrid-flux variance (held out) and coarse/fine spectral error",
                "value": r["a_priori_explained_variance"]["closure_mean"], "unit": "fraction", "n_gpus": 1,
                "steps": None, "warmup": None, "ms_per_step": r["a_posteriori"]["closure"]["ms_per_step"],

@@ -55,10 +55,26 @@ static int launch_group(const SpecLaunch& L, int compute_f32, hipStream_t s)
 }
 
 // The load operation is a compile-time parameter of the kernels, so jobs are
-// launched in groups of equal load operation.
+// launched in groups of equal load operation -- except first-derivative jobs
+// of the plain, product and Maxwell loads on the register-blocked lengths,
+// which share one launch of the run-time-dispatch kernel (LD_ANY): one
+// larger launch streams at a higher rate than three small ones.
 int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
 {
     if (L.njobs <= 0) return 0;
+    if (L.mode == SM_DERIV1 && L.njobs > 1 && spectral_len_fixed(L.dir == 0 ? L.g.nx : L.g.ny)) {
+        bool mergeable = true, mixed_ops = false;
+        for (int i = 0; i < L.njobs; ++i) {
+            const int op = L.job[i].load_op;
+            mergeable = mergeable && (op == LD_PLAIN || op == LD_PROD || op == LD_MAXW);
+            mixed_ops = mixed_ops || op != L.job[0].load_op;
+        }
+        if (mergeable && mixed_ops) {
+            SpecLaunch G = L;
+            G.any_load = 1;
+            return launch_group(G, compute_f32, s);
+        }
+    }
     bool done[SPEC_MAX_JOBS] = {};
     for (int i = 0; i < L.njobs; ++i) {
         if (done[i]) continue;

@@ -20,7 +20,11 @@ enum MulKind { MUL_D1 = 0, MUL_D2 = 1, MUL_MASK = 2, MUL_D2C = 3 };
 //   LD_PROD  scale*(in0+m0)*(in1+m1)
 //   LD_CROSS (in0+m0)*(in1+m1) - (in2+m2)*(in3+m3)
 //   LD_MAXW  scale*((in0+m0)^2 - (in1+m1)^2 - (in2+m2)^2)
-enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3, LD_MAXW = 4 };
+enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3, LD_MAXW = 4,
+              // kernel template value only: each job's own load_op (PLAIN, PROD
+              // or MAXW) chosen at run time, so first-derivative jobs of
+              // different load operations share one launch
+              LD_ANY = 5 };
 
 constexpr int SPEC_MAX_JOBS = 8;
 // lengths without a register-blocked plan: run-time radix plans up to this length
@@ -62,6 +66,7 @@ struct SpecLaunch {
     int variant;      // radix plan: 0 the default, 1 the length's alternative (if any)
     int max_groups;   // optional further cap on the grid; 0 = none
     int group_major;  // set by the launcher: items ordered line group first (see spec_kernel)
+    int any_load;     // set by the launcher: jobs of mixed load operations in one LD_ANY launch
     // set by the launcher for lengths without a register-blocked plan: the
     // run-time radices and the complex lines per workgroup
     int gen_radix[SPEC_GEN_MAX_PASSES];

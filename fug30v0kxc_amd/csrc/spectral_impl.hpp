@@ -306,15 +306,26 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body(const Spe
                 }
             }
         } else {
-            const Loader<TC, LOP> ld(J, ok ? k : 0);
+            auto load_with = [&](auto op) {
+                constexpr int LO = decltype(op)::value;
+                const Loader<TC, LO> ld(J, ok ? k : 0);
 #pragma unroll
-            for (int r = 0; r < R0; ++r) {
-                const int n = j + r * NB0;
-                dst[r] = {(TC)0, (TC)0};
-                if (ok && j < NB0) {
-                    if constexpr (DIR == 1) dst[r] = ld.pair(ia + n * st);
-                    else dst[r] = {ld(ia + n * st), ld(ib + n * st)};
+                for (int r = 0; r < R0; ++r) {
+                    const int n = j + r * NB0;
+                    dst[r] = {(TC)0, (TC)0};
+                    if (ok && j < NB0) {
+                        if constexpr (DIR == 1) dst[r] = ld.pair(ia + n * st);
+                        else dst[r] = {ld(ia + n * st), ld(ib + n * st)};
+                    }
                 }
+            };
+            if constexpr (LOP == LD_ANY) {
+                // the job's load operation, uniform over the workgroup
+                if (J.load_op == LD_PROD) load_with(std::integral_constant<int, LD_PROD>());
+                else if (J.load_op == LD_MAXW) load_with(std::integral_constant<int, LD_MAXW>());
+                else load_with(std::integral_constant<int, LD_PLAIN>());
+            } else {
+                load_with(std::integral_constant<int, LOP>());
             }
         }
     };
@@ -503,6 +514,7 @@ static int launch_one(const SpecLaunch& L, hipStream_t s)
 template <typename TC, typename PL, int DIR, int B>
 static int launch_mode(const SpecLaunch& L, hipStream_t s)
 {
+    if (L.any_load) return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_ANY>(L, s);
     const int lop = L.job[0].load_op;
     switch (L.mode * 8 + lop) {
     case SM_DERIV1 * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_PLAIN>(L, s);
@@ -524,6 +536,7 @@ static int launch_mode(const SpecLaunch& L, hipStream_t s)
 template <typename TC, typename PL, int B>
 static int launch_y_deriv1(const SpecLaunch& L, hipStream_t s)
 {
+    if (L.any_load) return launch_one<TC, PL, 1, B, SM_DERIV1, LD_ANY>(L, s);
     switch (L.job[0].load_op) {
     case LD_PLAIN: return launch_one<TC, PL, 1, B, SM_DERIV1, LD_PLAIN>(L, s);
     case LD_PROD: return launch_one<TC, PL, 1, B, SM_DERIV1, LD_PROD>(L, s);

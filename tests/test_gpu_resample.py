@@ -8,7 +8,7 @@ from oracle import pyoracle
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(3, 8, 12), (4, 64, 64), (2, 30, 20), (5, 96, 128)])
+@pytest.mark.parametrize("shape", [(3, 8, 12), (4, 64, 64), (2, 30, 20), (5, 96, 128), (2, 14, 10)])
 def test_expand_matches_oracle(shape):
     import fug30v0kxc_amd as fa
     print(f"[resample] expand {shape}", flush=True)
@@ -46,7 +46,7 @@ def test_errors():
         fa.expand(np.zeros((2, 8, 7)))
     assert e.value.code == -1
     with pytest.raises(fa.MhdError) as e:
-        fa.expand(np.zeros((2, 14, 8)))            # 14 has no transform kernel
+        fa.expand(np.zeros((2, 1026, 8)))          # 2 ny = 2052 is past the longest transform
     assert e.value.code == -20
     with pytest.raises(fa.MhdError):
         fa.box_filter(np.zeros((8, 16, 30)), 4)
