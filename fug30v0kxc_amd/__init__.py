@@ -337,7 +337,7 @@ class Solver:
         self._chk(self._lib.mhd_autotune(self._ctx, ctypes.byref(t)), "autotune")
         return {n: getattr(t, n) for n, _ in _abi.MhdTune._fields_}
 
-    def set_tune(self, variant_x=0, variant_y=0, mult_x=1, mult_y=1, zc=16):
+    def set_tune(self, variant_x=0, variant_y=0, mult_x=1, mult_y=1, zc=8):
         """Apply launch-shape choices (as returned by autotune)."""
         t = _abi.MhdTune(variant_x=variant_x, variant_y=variant_y, mult_x=mult_x, mult_y=mult_y, zc=zc)
         self._chk(self._lib.mhd_set_tune(self._ctx, ctypes.byref(t)), "set_tune")

@@ -103,6 +103,7 @@ struct mhd_ctx {
     int spec_var[2];   // radix plan per direction (0 default, 1 alternative), tuned
     int spec_mult[2];  // persistent-grid multiple per direction, tuned
     int hzc;           // planes per thread of the column-walking hydro kernel
+    int mzc;           // ... of the magnetic column kernels (mag_rhs, jsq)
     double* hbuf;      // pinned host staging (one field)
 
     mhd_scalars_t sc;
@@ -396,7 +397,7 @@ MagArgs margs(mhd_ctx* c)
     if (c->su) { a.bxm = c->bxbar; a.bym = c->bybar; }
     if (c->eta_on) { a.jx = c->jx; a.jy = c->jy; a.jz = c->jz; }
     a.wf32 = c->wf32;
-    a.zc = c->hzc;
+    a.zc = c->mzc;
     return a;
 }
 
@@ -414,6 +415,32 @@ MagArgs margs(mhd_ctx* c)
 // (new state in the ping-pong buffers), or null for the right-hand sides only
 int run_closure_flux(mhd_ctx* c);
 int run_closure_tend(mhd_ctx* c);
+
+// The AB3 update of u, v, w, rho, T fused into hydro_col: coefficients cf,
+// history levels 1 and 2 of the ring, the new state into the ping-pong
+// buffers; hz1, hz2 are the rho history levels the density floor clears.
+void fuse_ab3(mhd_ctx* c, HydroColArgs& a, const double* cf, void* hz1, void* hz2)
+{
+    a.ab3 = 1;
+    a.c0 = cf[0]; a.c1 = cf[1]; a.c2 = cf[2];
+    const int hf[5] = {0, 1, 2, 3, 4};   // u v w r t
+    for (int i = 0; i < 5; ++i) {
+        a.h1[i] = level_ptr(c, hf[i], 1);
+        a.h2[i] = level_ptr(c, hf[i], 2);
+    }
+    a.h3z[0] = hz1;
+    a.h3z[1] = hz2;
+    a.h3z[2] = level_ptr(c, 3, 3);
+    const double* x64[5] = {c->u, c->v, c->w, c->r, c->t};
+    double* xn[5] = {c->u2, c->v2, c->w2, c->r2, c->t2};
+    float* sn[5] = {c->su2, c->sv2, c->sw2, c->sr2, c->st2};
+    for (int i = 0; i < 5; ++i) {
+        a.x64[i] = x64[i];
+        a.xnew[i] = xn[i];
+        a.snew[i] = sn[i];
+    }
+    if (c->su) { a.t0 = c->t0; a.r0 = c->r0; }
+}
 
 int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
 {
@@ -494,27 +521,7 @@ int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
         TRY(spec(c, M));
     }
     HydroColArgs a = colargs(c);
-    if (cf) {
-        a.ab3 = 1;
-        a.c0 = cf[0]; a.c1 = cf[1]; a.c2 = cf[2];
-        const int hf[5] = {0, 1, 2, 3, 4};   // u v w r t
-        for (int i = 0; i < 5; ++i) {
-            a.h1[i] = level_ptr(c, hf[i], 1);
-            a.h2[i] = level_ptr(c, hf[i], 2);
-        }
-        a.h3z[0] = level_ptr(c, 3, 1);
-        a.h3z[1] = level_ptr(c, 3, 2);
-        a.h3z[2] = level_ptr(c, 3, 3);
-        const double* x64[5] = {c->u, c->v, c->w, c->r, c->t};
-        double* xn[5] = {c->u2, c->v2, c->w2, c->r2, c->t2};
-        float* sn[5] = {c->su2, c->sv2, c->sw2, c->sr2, c->st2};
-        for (int i = 0; i < 5; ++i) {
-            a.x64[i] = x64[i];
-            a.xnew[i] = xn[i];
-            a.snew[i] = sn[i];
-        }
-        if (c->su) { a.t0 = c->t0; a.r0 = c->r0; }
-    }
+    if (cf) fuse_ab3(c, a, cf, level_ptr(c, 3, 1), level_ptr(c, 3, 2));
     TRY(timed(c, MHD_K_HYDRO, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); }));
     return c->closure ? run_closure_tend(c) : 0;
 }
@@ -1011,14 +1018,16 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     c->ncu = ncu;
     // launch shapes until mhd_autotune / mhd_set_tune: the autotuner's choices at
     // 256x256x160 on MI355X (y transforms with two resident grids in FP64; 8
-    // planes per column-walk thread in FP64, 12 in the mixed mode)
+    // planes per column-walk thread)
     c->spec_var[0] = c->spec_var[1] = 0;
     c->spec_mult[0] = 1;
     c->spec_mult[1] = c->wf32 ? 1 : 2;
     {
         const int nin = c->nz > 8 ? c->nz - 8 : 1;   // interior planes
-        const int zc = c->wf32 ? 12 : 8;
-        c->hzc = zc < nin ? zc : nin;
+        c->hzc = 8 < nin ? 8 : nin;
+        // mag_rhs and jsq: 8 planes measured fastest at 256x256x160 (8 to 76
+        // tried; 24 planes cost 30-35 us more per step in both precisions)
+        c->mzc = 8 < c->nz ? 8 : c->nz;
     }
 
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return MHD_E_DEVICE; }
@@ -1858,6 +1867,13 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
         if (zc > nin && zc != cands[0]) continue;
         c->hzc = zc;
         HydroColArgs a = colargs(c);
+        if (!c->closure) {
+            // the variant the step runs: with the fused AB3 update (zero
+            // coefficients; the new state goes to the free ping-pong buffers
+            // and the floor's history clears to a free work array)
+            const double cf0[3] = {0.0, 0.0, 0.0};
+            fuse_ab3(c, a, cf0, W[W_RX], W[W_RX]);
+        }
         double ms = 0.0;
         TRY(time_ms(c, 3, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); }, &ms));
         if (ms < bestc) { bestc = ms; bz = zc; }

@@ -54,14 +54,40 @@ static int launch_group(const SpecLaunch& L, int compute_f32, hipStream_t s)
     return launch_spectral_g5(n, L, compute_f32, s);
 }
 
+// Three LD_CROSS jobs whose factors are the components of two vectors a, b
+// with job q = a_{q+1} b_{q+2} - a_{q+2} b_{q+1}: the EMF u x B.
+static bool is_cross_product(const SpecLaunch& L)
+{
+    if (L.njobs != 3) return false;
+    const SpecJob* J = L.job;
+    for (int q = 0; q < 3; ++q)
+        if (J[q].load_op != LD_CROSS || J[q].in_f32 != J[0].in_f32) return false;
+    // a_{q+1} = J[q].in[0], b_{q+2} = J[q].in[1]
+    const void *a[3], *b[3];
+    for (int q = 0; q < 3; ++q) {
+        a[(q + 1) % 3] = J[q].in[0];
+        b[(q + 2) % 3] = J[q].in[1];
+    }
+    for (int q = 0; q < 3; ++q)
+        if (J[q].in[2] != a[(q + 2) % 3] || J[q].in[3] != b[(q + 1) % 3]) return false;
+    return true;
+}
+
 // The load operation is a compile-time parameter of the kernels, so jobs are
 // launched in groups of equal load operation -- except first-derivative jobs
 // of the plain, product and Maxwell loads on the register-blocked lengths,
 // which share one launch of the run-time-dispatch kernel (LD_ANY): one
-// larger launch streams at a higher rate than three small ones.
+// larger launch streams at a higher rate than three small ones; and the
+// three components of a cross product along x on those lengths, which share
+// their six factor loads (LD_CROSS3).
 int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
 {
     if (L.njobs <= 0) return 0;
+    if (L.mode == SM_R2HC && L.dir == 0 && spectral_len_fixed(L.g.nx) && is_cross_product(L)) {
+        SpecLaunch G = L;
+        G.cross3 = 1;
+        return launch_group(G, compute_f32, s);
+    }
     if (L.mode == SM_DERIV1 && L.njobs > 1 && spectral_len_fixed(L.dir == 0 ? L.g.nx : L.g.ny)) {
         bool mergeable = true, mixed_ops = false;
         for (int i = 0; i < L.njobs; ++i) {

@@ -24,7 +24,12 @@ enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3, LD_MAXW = 4,
               // kernel template value only: each job's own load_op (PLAIN, PROD
               // or MAXW) chosen at run time, so first-derivative jobs of
               // different load operations share one launch
-              LD_ANY = 5 };
+              LD_ANY = 5,
+              // kernel template value only: three LD_CROSS jobs that are the
+              // components of one cross product a x b, job q loading
+              // {a_{q+1}, b_{q+2}, a_{q+2}, b_{q+1}} (indices mod 3); the six
+              // factor lines of a line group are loaded once for all three
+              LD_CROSS3 = 6 };
 
 constexpr int SPEC_MAX_JOBS = 8;
 // lengths without a register-blocked plan: run-time radix plans up to this length
@@ -67,6 +72,7 @@ struct SpecLaunch {
     int max_groups;   // optional further cap on the grid; 0 = none
     int group_major;  // set by the launcher: items ordered line group first (see spec_kernel)
     int any_load;     // set by the launcher: jobs of mixed load operations in one LD_ANY launch
+    int cross3;       // set by the launcher: the three jobs form one cross product (LD_CROSS3)
     // set by the launcher for lengths without a register-blocked plan: the
     // run-time radices and the complex lines per workgroup
     int gen_radix[SPEC_GEN_MAX_PASSES];

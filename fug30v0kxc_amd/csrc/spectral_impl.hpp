@@ -121,16 +121,27 @@ struct Loader {
                     scale * x.y * x.y - scale * y.y * y.y - scale * z.y * z.y +
                         (TC)2 * scale * (m0 * x.y - m1 * y.y - m2 * z.y)};
         } else {
-            const cpx<TC> x = ld_pair(a, i), y = ld_pair(b, i), z = ld_pair(c, i), w = ld_pair(d, i);
-            cpx<TC> r = {x.x * y.x - z.x * w.x + (x.x * m1 + m0 * y.x) - (z.x * m3 + m2 * w.x),
-                         x.y * y.y - z.y * w.y + (x.y * m1 + m0 * y.y) - (z.y * m3 + m2 * w.y)};
-            if (jj) {
-                const cpx<TC> q = ld_pair(jj, i);
-                r.x = r.x - jc * q.x;
-                r.y = r.y - jc * q.y;
-            }
-            return r;
+            return cross_pair(ld_pair(a, i), ld_pair(b, i), ld_pair(c, i), ld_pair(d, i), i);
         }
+    }
+    // LD_CROSS from factor values already loaded (x = in0, y = in1, z = in2,
+    // w = in3 at element i), the extra term loaded here
+    __device__ __forceinline__ TC cross(TC x, TC y, TC z, TC w, long i) const
+    {
+        TC r = x * y - z * w + (x * m1 + m0 * y) - (z * m3 + m2 * w);
+        if (jj) r = r - jc * jj[i];
+        return r;
+    }
+    __device__ __forceinline__ cpx<TC> cross_pair(cpx<TC> x, cpx<TC> y, cpx<TC> z, cpx<TC> w, long i) const
+    {
+        cpx<TC> r = {x.x * y.x - z.x * w.x + (x.x * m1 + m0 * y.x) - (z.x * m3 + m2 * w.x),
+                     x.y * y.y - z.y * w.y + (x.y * m1 + m0 * y.y) - (z.y * m3 + m2 * w.y)};
+        if (jj) {
+            const cpx<TC> q = ld_pair(jj, i);
+            r.x = r.x - jc * q.x;
+            r.y = r.y - jc * q.y;
+        }
+        return r;
     }
     __device__ __forceinline__ TC operator()(long i) const
     {
@@ -145,10 +156,7 @@ struct Loader {
             return scale * x * x - scale * y * y - scale * z * z +
                    (TC)2 * scale * (m0 * x - m1 * y - m2 * z);
         } else {
-            const TC x = a[i], y = b[i], z = c[i], w = d[i];
-            TC r = x * y - z * w + (x * m1 + m0 * y) - (z * m3 + m2 * w);
-            if (jj) r = r - jc * jj[i];
-            return r;
+            return cross(a[i], b[i], c[i], d[i], i);
         }
     }
 };
@@ -229,15 +237,145 @@ struct Lines {
 // r G + i mod G: a workgroup then does every job of one line group back to
 // back, so jobs that load the same fields (the cross products of u and B)
 // find the lines in their CU's caches instead of re-reading HBM.
+//
+// LOP = LD_CROSS3 (SM_R2HC, three jobs forming one cross product) runs
+// body_cross3 instead: an item is one line group, whose six factor lines are
+// loaded once and combined into the three jobs' inputs.
 template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
 struct SpecK {
     using Args = SpecLaunch;
     static constexpr int NT = B * plan_tpt<PL>();
-    static __device__ __forceinline__ void body(const SpecLaunch& L, uint3 bid, uint3 gdim);
+    static __device__ __forceinline__ void body(const SpecLaunch& L, uint3 bid, uint3 gdim)
+    {
+        if constexpr (LOP == LD_CROSS3) body_cross3(L, bid, gdim);
+        else body_jobs(L, bid, gdim);
+    }
+    static __device__ __forceinline__ void body_jobs(const SpecLaunch& L, uint3 bid, uint3 gdim);
+    static __device__ __forceinline__ void body_cross3(const SpecLaunch& L, uint3 bid, uint3 gdim);
+    // forward spectrum in v -> the two halfcomplex lines of the pair, stored
+    static __device__ __forceinline__ void r2hc_store(cpx<TC>* v, int j, cpx<TC>* lds, bool ok, TC* __restrict__ out0,
+                                                      long ia, long ib, long st);
 };
 
 template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
-__device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body(const SpecLaunch& L, uint3 bid, uint3 gdim)
+__device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::r2hc_store(cpx<TC>* v, int j, cpx<TC>* lds, bool ok,
+                                                                             TC* __restrict__ out0, long ia, long ib,
+                                                                             long st)
+{
+    constexpr int N = PL::N;
+    using F = RegFFT<TC, PL>;
+    constexpr int NB0 = F::NB0, NBL = F::NBL, R0 = F::R0, RL = F::RL;
+    // spectrum -> LDS, then separate the pair into two halfcomplex lines
+    if (j < NBL) {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) lds[F::pad(j + r * NBL)] = v[r];
+    }
+    __syncthreads();
+    if (ok && j < NB0) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int n = j + r * NB0;
+            const int kk = (n <= N / 2) ? n : N - n;
+            const cpx<TC> zk = lds[F::pad(kk)];
+            const cpx<TC> zn = lds[F::pad((N - kk) % N)];
+            TC a, b;
+            if (n <= N / 2) { a = (TC)0.5 * (zk.x + zn.x); b = (TC)0.5 * (zk.y + zn.y); }
+            else { a = (TC)0.5 * (zk.y - zn.y); b = (TC)0.5 * (zn.x - zk.x); }
+            if constexpr (DIR == 1) st_pair(out0, ia + n * st, a, b);
+            else {
+                out0[ia + n * st] = a;
+                out0[ib + n * st] = b;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
+__device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_cross3(const SpecLaunch& L, uint3 bid,
+                                                                              uint3 gdim)
+{
+    constexpr int N = PL::N;
+    using F = RegFFT<TC, PL>;
+    using LN = Lines<PL, DIR, B>;
+    constexpr int NB0 = F::NB0, R0 = F::R0;
+    const Geom g = L.g;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<TC>* twl = reinterpret_cast<cpx<TC>*>(smem);
+    const cpx<TC>* __restrict__ twg = reinterpret_cast<const cpx<TC>*>(L.tw);
+    for (int i = threadIdx.x; i < N; i += NT) twl[i] = twg[i];
+    int t, j;
+    LN::lane(threadIdx.x, t, j);
+    cpx<TC>* lds = twl + N + t * F::SP;
+    const long ngrp = LN::groups(g);
+    const long G = gdim.x;
+    // the factors: job q loads {a_{q+1}, b_{q+2}, a_{q+2}, b_{q+1}}
+    const TC* __restrict__ a0 = (const TC*)L.job[2].in[0];
+    const TC* __restrict__ a1 = (const TC*)L.job[0].in[0];
+    const TC* __restrict__ a2 = (const TC*)L.job[1].in[0];
+    const TC* __restrict__ b0 = (const TC*)L.job[1].in[1];
+    const TC* __restrict__ b1 = (const TC*)L.job[2].in[1];
+    const TC* __restrict__ b2 = (const TC*)L.job[0].in[1];
+    __syncthreads();
+
+    // the three jobs' pass-0 inputs of line group grp
+    auto fetch = [&](long grp, cpx<TC> (*dst)[R0]) {
+        long ia, ib, st;
+        int k;
+        const bool ok = LN::base(g, grp, t, ia, ib, st, k);
+        const Loader<TC, LD_CROSS> l0(L.job[0], ok ? k : 0), l1(L.job[1], ok ? k : 0), l2(L.job[2], ok ? k : 0);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int n = j + r * NB0;
+            dst[0][r] = dst[1][r] = dst[2][r] = {(TC)0, (TC)0};
+            if (ok && j < NB0) {
+                if constexpr (DIR == 1) {
+                    const long i = ia + n * st;
+                    const cpx<TC> x0 = ld_pair(a0, i), x1 = ld_pair(a1, i), x2 = ld_pair(a2, i);
+                    const cpx<TC> y0 = ld_pair(b0, i), y1 = ld_pair(b1, i), y2 = ld_pair(b2, i);
+                    dst[0][r] = l0.cross_pair(x1, y2, x2, y1, i);
+                    dst[1][r] = l1.cross_pair(x2, y0, x0, y2, i);
+                    dst[2][r] = l2.cross_pair(x0, y1, x1, y0, i);
+                } else {
+                    const long i = ia + n * st, e = ib + n * st;
+                    const TC x0 = a0[i], x1 = a1[i], x2 = a2[i], y0 = b0[i], y1 = b1[i], y2 = b2[i];
+                    const TC u0 = a0[e], u1 = a1[e], u2 = a2[e], w0 = b0[e], w1 = b1[e], w2 = b2[e];
+                    dst[0][r] = {l0.cross(x1, y2, x2, y1, i), l0.cross(u1, w2, u2, w1, e)};
+                    dst[1][r] = {l1.cross(x2, y0, x0, y2, i), l1.cross(u2, w0, u0, w2, e)};
+                    dst[2][r] = {l2.cross(x0, y1, x1, y0, i), l2.cross(u0, w1, u1, w0, e)};
+                }
+            }
+        }
+    };
+
+    cpx<TC> nxt[3][R0];
+    long grp = bid.x;
+    if (grp < ngrp) fetch(grp, nxt);
+    for (; grp < ngrp; grp += G) {
+        cpx<TC> cur[3][R0];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) cur[q][r] = nxt[q][r];
+        long ia, ib, st;
+        int k;
+        const bool ok = LN::base(g, grp, t, ia, ib, st, k);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            // the next group's loads fly during the last job's transform
+            if (q == 2 && grp + G < ngrp) fetch(grp + G, nxt);
+            cpx<TC> v[F::RMAX];
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[r] = cur[q][r];
+            F::forward(v, j, lds, twl);
+            r2hc_store(v, j, lds, ok, (TC*)L.job[q].out[0], ia, ib, st);
+        }
+    }
+}
+
+template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
+__device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_jobs(const SpecLaunch& L, uint3 bid,
+                                                                            uint3 gdim)
 {
     constexpr int N = PL::N;
     using F = RegFFT<TC, PL>;
@@ -388,30 +526,7 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body(const Spe
         } else {
             F::forward(v, j, lds, twl);
             if constexpr (MODE == SM_R2HC) {
-                // spectrum -> LDS, then separate the pair into two halfcomplex lines
-                if (j < NBL) {
-#pragma unroll
-                    for (int r = 0; r < RL; ++r) lds[F::pad(j + r * NBL)] = v[r];
-                }
-                __syncthreads();
-                if (ok && j < NB0) {
-#pragma unroll
-                    for (int r = 0; r < R0; ++r) {
-                        const int n = j + r * NB0;
-                        const int kk = (n <= N / 2) ? n : N - n;
-                        const cpx<TC> zk = lds[F::pad(kk)];
-                        const cpx<TC> zn = lds[F::pad((N - kk) % N)];
-                        TC a, b;
-                        if (n <= N / 2) { a = (TC)0.5 * (zk.x + zn.x); b = (TC)0.5 * (zk.y + zn.y); }
-                        else { a = (TC)0.5 * (zk.y - zn.y); b = (TC)0.5 * (zn.x - zk.x); }
-                        if constexpr (DIR == 1) st_pair(out0, ia + n * st, a, b);
-                        else {
-                            out0[ia + n * st] = a;
-                            out0[ib + n * st] = b;
-                        }
-                    }
-                }
-                __syncthreads();
+                r2hc_store(v, j, lds, ok, out0, ia, ib, st);
             } else {
                 constexpr int NOUT = (MODE == SM_DERIV2) ? 2 : 1;
                 cpx<TC> z[RL];
@@ -482,7 +597,8 @@ static int launch_one(const SpecLaunch& L, hipStream_t s)
     constexpr size_t lds = spec_lds_bytes<TC, PL, B>();
     static_assert(lds <= (size_t)SPEC_LDS_Y, "transform workgroup exceeds 160 KB of LDS");
     auto kern = single_kernel<K>;
-    const long items = Lines<PL, DIR, B>::groups(L.g) * L.njobs;
+    // LD_CROSS3: one item per line group for all three jobs
+    const long items = Lines<PL, DIR, B>::groups(L.g) * (LOP == LD_CROSS3 ? 1 : L.njobs);
     // more than 64 KB of dynamic LDS is opted into once per kernel
     static int lds_ok = lds <= 65536 ? 1 : 0;
     if (!lds_ok) {
@@ -515,6 +631,9 @@ template <typename TC, typename PL, int DIR, int B>
 static int launch_mode(const SpecLaunch& L, hipStream_t s)
 {
     if (L.any_load) return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_ANY>(L, s);
+    if constexpr (DIR == 0) {
+        if (L.cross3) return launch_one<TC, PL, DIR, B, SM_R2HC, LD_CROSS3>(L, s);
+    }
     const int lop = L.job[0].load_op;
     switch (L.mode * 8 + lop) {
     case SM_DERIV1 * 8 + LD_PLAIN: return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_PLAIN>(L, s);
@@ -551,7 +670,9 @@ static int launch_plan(const SpecLaunch& L, hipStream_t s)
     if (L.dir == 0) return launch_mode<TC, PL, 0, spec_b<TC, PL, 0>()>(L, s);
     constexpr int B0 = spec_b<TC, PL, 1>();
     // FP32 only: at 256 points 8 pairs beat 16 for these passes (76, 72, 45 us
-    // against 81, 83, 48); in FP64 4 pairs lose to 8
+    // against 81, 83, 48); in FP64 4 pairs lose to 8.  The other FP32 modes keep
+    // 16 pairs (8 pairs: d1+d2 155 against 137 us, hc2r 125 against 109, r2hc
+    // 63 against 58)
     constexpr int BH = (sizeof(TC) == 4 && B0 >= 8) ? B0 / 2 : B0;
     if (BH != B0 && L.mode == SM_DERIV1 && L.g.nx % (2 * BH) == 0) return launch_y_deriv1<TC, PL, BH>(L, s);
     if (L.g.nx % (2 * B0) == 0) return launch_mode<TC, PL, 1, B0>(L, s);

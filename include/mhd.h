@@ -202,8 +202,11 @@ long mhd_dump_header(const mhd_dump_t* meta, int iprec, unsigned char* buf, long
  * context: per transform direction the radix plan (default or the length's
  * alternative) and the persistent-grid multiple (1 or 2 x resident), on the
  * hydro d/dx, d2/dx2 launch; the planes per thread of the column-walking
- * kernels.  Needs a state; overwrites only work arrays and the newest RHS
- * level (recomputed by every step), so it may run between steps. */
+ * hydro kernel, in the variant the step runs (with the fused AB3 update
+ * unless a closure is attached; the magnetic column kernels keep 8 planes).
+ * Needs a state; overwrites only work arrays, the buffers the next step's
+ * new state goes to and the newest RHS level (all rewritten by every step
+ * before they are read), so it may run between steps. */
 typedef struct mhd_tune {
     int variant_x, variant_y, mult_x, mult_y, zc;
     double ms_x, ms_y, ms_col;   /* best times of the timed launches */

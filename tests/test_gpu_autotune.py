@@ -56,6 +56,29 @@ def test_autotune_then_steps_match_oracle():
     s.close()
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5)])
+def test_autotune_between_steps_matches_oracle(prec, tol):
+    """The tuner runs the fused hydro kernel into the buffers the next step
+    writes; tuning between steps leaves the run unchanged."""
+    from fug30v0kxc_amd import Solver
+    print(f"[autotune] tune between steps 64x64x48 precision {prec}", flush=True)
+    p = reference_params(t_amp=1e-3, b_amp=1e-3)
+    s = Solver(64, 64, 48, p, precision=prec)
+    s.static()
+    o = Oracle(64, 64, 48, p)
+    o.setup_static()
+    s.step(5, 1e-4, 0)
+    s.autotune()
+    s.step(7, 1e-4, 0)
+    o.step(12, 1e-4, 0)
+    st = s.get_state()
+    bscale = max(np.abs(o[n]).max() for n in ("bx", "by", "bz"))
+    for n in NAMES:
+        scale = bscale if n.startswith("b") else np.abs(o[n]).max()
+        assert np.abs(st[n] - o[n]).max() / scale < tol, n
+    s.close()
+
+
 def test_forced_alternative_steps_match_oracle():
     from fug30v0kxc_amd import MHD_PREC_MIXED, Solver
     print("[autotune] forced alternatives, steps 64x64x48, both precisions", flush=True)
