@@ -79,6 +79,14 @@ template <> struct AltPlan<128> { using type = Plan<128, 8, 16>; };
 template <> struct AltPlan<160> { using type = Plan<160, 4, 8, 5>; };
 template <> struct AltPlan<256> { using type = Plan<256, 4, 8, 8>; };
 template <> struct AltPlan<512> { using type = Plan<512, 8, 8, 8>; };
+// a second alternative (SpecLaunch::variant = 2): two radix-16 passes, 16
+// threads per transform, so a workgroup of 16 FP32 column pairs has 256
+// threads instead of 1024 and several fit on a CU
+template <int N>
+struct AltPlan2 {
+    using type = void;
+};
+template <> struct AltPlan2<256> { using type = Plan<256, 16, 16>; };
 
 template <typename PL>
 constexpr int plan_radix(int p, bool inv)

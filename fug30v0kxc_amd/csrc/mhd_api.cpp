@@ -100,7 +100,7 @@ struct mhd_ctx {
     double *red_partial, *red_result, *d_bzbar;
     double* stage;     // device FP64 staging (one field)
     int ncu;           // compute units (persistent transform grids)
-    int spec_var[2];   // radix plan per direction (0 default, 1 alternative), tuned
+    int spec_var[2][4]; // radix plan per direction and transform kind (SpecMode), tuned
     int spec_mult[2];  // persistent-grid multiple per direction, tuned
     int hzc;           // planes per thread of the column-walking hydro kernel
     int mzc;           // ... of the magnetic column kernels (mag_rhs, jsq)
@@ -251,10 +251,14 @@ SpecLaunch spec_base(mhd_ctx* c, int dir, int mode, bool f64 = false)
     else L.tw = dir == 0 ? c->twx : c->twy;
     L.icut = dir == 0 ? c->icutx : c->icuty;
     L.ncu = c->ncu;
-    L.variant = c->spec_var[dir];
+    L.variant = c->spec_var[dir][mode];
     L.grid_mult = c->spec_mult[dir];
     return L;
 }
+
+static_assert(MHD_TK_DERIV1 == SM_DERIV1 && MHD_TK_DERIV2 == SM_DERIV2 && MHD_TK_R2HC == SM_R2HC &&
+                  MHD_TK_HC2R == SM_HC2R,
+              "the tuner's transform kinds index the context's plans by SpecMode");
 
 SpecJob sjob(const void* in, int in_f32, void* o0, void* o1, int m0, int m1, int out_f32)
 {
@@ -1019,7 +1023,7 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     // launch shapes until mhd_autotune / mhd_set_tune: the autotuner's choices at
     // 256x256x160 on MI355X (y transforms with two resident grids in FP64; 8
     // planes per column-walk thread)
-    c->spec_var[0] = c->spec_var[1] = 0;
+    for (int k = 0; k < 4; ++k) c->spec_var[0][k] = c->spec_var[1][k] = 0;
     c->spec_mult[0] = 1;
     c->spec_mult[1] = c->wf32 ? 1 : 2;
     {
@@ -1810,12 +1814,15 @@ int mhd_set_tune(mhd_ctx_t* c, const mhd_tune_t* t)
 {
     TRY(check_ctx(c));
     if (!t) return MHD_E_ARG;
-    if (t->variant_x < 0 || t->variant_x > 1 || t->variant_y < 0 || t->variant_y > 1) return MHD_E_ARG;
-    if ((t->variant_x && !spectral_has_variant(c->nx)) || (t->variant_y && !spectral_has_variant(c->ny)))
-        return MHD_E_ARG;
+    for (int k = 0; k < MHD_TK_N; ++k)
+        if (t->variant_x[k] < 0 || t->variant_y[k] < 0 || t->variant_x[k] >= spectral_variants(c->nx) ||
+            t->variant_y[k] >= spectral_variants(c->ny))
+            return MHD_E_ARG;
     if (t->mult_x < 1 || t->mult_x > 8 || t->mult_y < 1 || t->mult_y > 8 || t->zc < 1) return MHD_E_ARG;
-    c->spec_var[0] = t->variant_x;
-    c->spec_var[1] = t->variant_y;
+    for (int k = 0; k < MHD_TK_N; ++k) {
+        c->spec_var[0][k] = t->variant_x[k];
+        c->spec_var[1][k] = t->variant_y[k];
+    }
     c->spec_mult[0] = t->mult_x;
     c->spec_mult[1] = t->mult_y;
     c->hzc = t->zc;
@@ -1834,29 +1841,48 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
                          c->su ? (const void*)c->sv : c->v, c->su ? (const void*)c->sw : c->w};
     const int sf = c->su ? 1 : 0;
     double best_ms[2] = {0, 0};
-    // transforms: radix plan x persistent-grid multiple, on the hydro d/dx, d2/dx2 set
+    // transforms, per direction: the persistent-grid multiple on the hydro
+    // d/dx, d2/dx2 set with the default plans, then per transform kind the
+    // radix plan on a 4-field launch of that kind (outputs into work arrays)
+    const int o1[2][4] = {{W_TX, W_UX, W_VX, W_WX}, {W_TY, W_UY, W_VY, W_WY}};
+    const int o2[2][4] = {{W_TXX, W_UXX, W_VXX, W_WXX}, {W_TYY, W_UYY, W_VYY, W_WYY}};
+    auto kind_launch = [&](int dir, int kind) {
+        SpecLaunch L = spec_base(c, dir, kind);
+        for (int j = 0; j < 4; ++j) {
+            void* a = W[o1[dir][j]];
+            if (kind == SM_DERIV2) L.job[j] = sjob(in[j], sf, a, W[o2[dir][j]], MUL_D1, MUL_D2, wf);
+            else if (kind == SM_DERIV1) L.job[j] = sjob(in[j], sf, a, nullptr, MUL_D1, 0, wf);
+            else L.job[j] = sjob(in[j], sf, a, nullptr, 0, 0, wf);
+        }
+        L.njobs = 4;
+        return L;
+    };
     for (int dir = 0; dir < 2; ++dir) {
         const int n = dir == 0 ? c->nx : c->ny;
+        for (int k = 0; k < MHD_TK_N; ++k) c->spec_var[dir][k] = 0;
         double best = 1e30;
-        int bv = 0, bm = 1;
-        for (int var = 0; var < (spectral_has_variant(n) ? 2 : 1); ++var)
-            for (int mult = 1; mult <= 2; ++mult) {
-                c->spec_var[dir] = var;
-                c->spec_mult[dir] = mult;
-                SpecLaunch L = spec_base(c, dir, SM_DERIV2);
-                const int o1[4] = {dir == 0 ? W_TX : W_TY, dir == 0 ? W_UX : W_UY, dir == 0 ? W_VX : W_VY,
-                                   dir == 0 ? W_WX : W_WY};
-                const int o2[4] = {dir == 0 ? W_TXX : W_TYY, dir == 0 ? W_UXX : W_UYY,
-                                   dir == 0 ? W_VXX : W_VYY, dir == 0 ? W_WXX : W_WYY};
-                for (int j = 0; j < 4; ++j) L.job[j] = sjob(in[j], sf, W[o1[j]], W[o2[j]], MUL_D1, MUL_D2, wf);
-                L.njobs = 4;
+        int bm = 1;
+        for (int mult = 1; mult <= 2; ++mult) {
+            c->spec_mult[dir] = mult;
+            const SpecLaunch L = kind_launch(dir, SM_DERIV2);
+            double ms = 0.0;
+            TRY(time_ms(c, 3, [&] { return spec(c, L); }, &ms));
+            if (ms < best) { best = ms; bm = mult; }
+        }
+        c->spec_mult[dir] = bm;
+        for (int kind = 0; kind < MHD_TK_N; ++kind) {
+            double bk = 1e30;
+            int bv = 0;
+            for (int var = 0; var < spectral_variants(n); ++var) {
+                c->spec_var[dir][kind] = var;
+                const SpecLaunch L = kind_launch(dir, kind);
                 double ms = 0.0;
                 TRY(time_ms(c, 3, [&] { return spec(c, L); }, &ms));
-                if (ms < best) { best = ms; bv = var; bm = mult; }
+                if (ms < bk) { bk = ms; bv = var; }
             }
-        c->spec_var[dir] = bv;
-        c->spec_mult[dir] = bm;
-        best_ms[dir] = best;
+            c->spec_var[dir][kind] = bv;
+            if (kind == SM_DERIV2) best_ms[dir] = bk;
+        }
     }
     // column-walking kernels: planes per thread
     double bestc = 1e30;
@@ -1881,8 +1907,10 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
     c->hzc = bz;
     c->timed_kernel = saved_timing;
     if (out) {
-        out->variant_x = c->spec_var[0];
-        out->variant_y = c->spec_var[1];
+        for (int k = 0; k < MHD_TK_N; ++k) {
+            out->variant_x[k] = c->spec_var[0][k];
+            out->variant_y[k] = c->spec_var[1][k];
+        }
         out->mult_x = c->spec_mult[0];
         out->mult_y = c->spec_mult[1];
         out->zc = c->hzc;

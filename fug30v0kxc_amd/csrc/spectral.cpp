@@ -32,14 +32,16 @@ bool spectral_len_supported(int n)
     return generic_plan(n, r, &np);
 }
 
-bool spectral_has_variant(int n)
+int spectral_variants(int n)
 {
     switch (n) {
-#define X(n_) case n_: return !std::is_void<AltPlan<n_>::type>::value;
+#define X(n_)                                                                  \
+    case n_:                                                                   \
+        return std::is_void<AltPlan<n_>::type>::value ? 1 : std::is_void<AltPlan2<n_>::type>::value ? 2 : 3;
         MHD_ALL_SIZES(X)
 #undef X
     }
-    return false;
+    return 1;
 }
 
 static int launch_group(const SpecLaunch& L, int compute_f32, hipStream_t s)

@@ -68,7 +68,7 @@ struct SpecLaunch {
     int icut;         // dealias cut-off (n/2)/3
     int ncu;          // compute units: the persistent grid is occupancy x ncu workgroups
     int grid_mult;    // ... times this (>= 1; 0 = 1), a tunable launch shape
-    int variant;      // radix plan: 0 the default, 1 the length's alternative (if any)
+    int variant;      // radix plan: 0 the default, 1 and 2 the length's alternatives (if any)
     int max_groups;   // optional further cap on the grid; 0 = none
     int group_major;  // set by the launcher: items ordered line group first (see spec_kernel)
     int any_load;     // set by the launcher: jobs of mixed load operations in one LD_ANY launch
@@ -91,7 +91,7 @@ bool spectral_len_fixed(int n);
 // run-time radix plan of an even length (false: odd or too long)
 bool generic_plan(int n, int radix[SPEC_GEN_MAX_PASSES], int* npass);
 int launch_spectral_generic(const SpecLaunch& L, int compute_f32, hipStream_t s);
-// the length has an alternative radix plan (SpecLaunch::variant = 1)
-bool spectral_has_variant(int n);
+// radix plans of the length: SpecLaunch::variant runs from 0 to this - 1
+int spectral_variants(int n);
 
 }  // namespace mhd

@@ -685,6 +685,9 @@ static int launch_n(const SpecLaunch& L, hipStream_t s)
     if constexpr (!std::is_void<typename AltPlan<N>::type>::value) {
         if (L.variant == 1) return launch_plan<TC, typename AltPlan<N>::type>(L, s);
     }
+    if constexpr (!std::is_void<typename AltPlan2<N>::type>::value) {
+        if (L.variant == 2) return launch_plan<TC, typename AltPlan2<N>::type>(L, s);
+    }
     return launch_plan<TC, FftPlan<N>>(L, s);
 }
 

@@ -199,17 +199,23 @@ long mhd_dump_header(const mhd_dump_t* meta, int iprec, unsigned char* buf, long
 
 /* ---- autotuner (no counterpart in the reference) ---------------------------
  * Times this box's launch shapes on this GPU and keeps the fastest in the
- * context: per transform direction the radix plan (default or the length's
- * alternative) and the persistent-grid multiple (1 or 2 x resident), on the
- * hydro d/dx, d2/dx2 launch; the planes per thread of the column-walking
+ * context: per transform direction the persistent-grid multiple (1 or 2 x
+ * resident, on the hydro d/dx, d2/dx2 launch) and, per transform kind (first
+ * derivatives, first and second derivatives, real to halfcomplex and back),
+ * the radix plan (the default or one of the length's alternatives, on a
+ * launch of that kind); the planes per thread of the column-walking
  * hydro kernel, in the variant the step runs (with the fused AB3 update
  * unless a closure is attached; the magnetic column kernels keep 8 planes).
  * Needs a state; overwrites only work arrays, the buffers the next step's
  * new state goes to and the newest RHS level (all rewritten by every step
  * before they are read), so it may run between steps. */
+enum { MHD_TK_DERIV1 = 0, MHD_TK_DERIV2 = 1, MHD_TK_R2HC = 2, MHD_TK_HC2R = 3, MHD_TK_N = 4 };
 typedef struct mhd_tune {
-    int variant_x, variant_y, mult_x, mult_y, zc;
-    double ms_x, ms_y, ms_col;   /* best times of the timed launches */
+    /* radix plan per transform kind (index MHD_TK_*): 0 the default, 1 and 2
+     * the length's alternatives where it has them */
+    int variant_x[MHD_TK_N], variant_y[MHD_TK_N];
+    int mult_x, mult_y, zc;
+    double ms_x, ms_y, ms_col;   /* the tuned d/dx+d2/dx2, d/dy+d2/dy2 and hydro column launches */
 } mhd_tune_t;
 int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out);
 /* apply given choices (e.g. tuned on an earlier run); MHD_E_ARG if invalid */

@@ -44,7 +44,8 @@ def test_autotune_then_steps_match_oracle():
     s = Solver(64, 64, 48, p)
     s.static()
     t = s.autotune()
-    assert t["variant_x"] in (0, 1) and t["mult_x"] in (1, 2) and t["zc"] >= 8
+    assert all(v in (0, 1, 2) for v in t["variant_x"] + t["variant_y"])
+    assert t["mult_x"] in (1, 2) and t["zc"] >= 8
     assert t["ms_x"] > 0 and t["ms_col"] > 0
     o = Oracle(64, 64, 48, p)
     o.setup_static()
@@ -99,6 +100,47 @@ def test_forced_alternative_steps_match_oracle():
         s.close()
 
 
+@pytest.mark.parametrize("shape", [(256, 64, 10), (64, 256, 10)])
+def test_second_alternative_plan_matches_oracle(shape):
+    # 256 points have a second alternative plan (two radix-16 passes)
+    from fug30v0kxc_amd import Solver
+    nx, ny, nz = shape
+    print(f"[autotune] second alternative plan {shape}", flush=True)
+    p = reference_params()
+    s = Solver(nx, ny, nz, p)
+    s.set_tune(variant_x=2 if nx == 256 else 0, variant_y=2 if ny == 256 else 0, mult_x=1, mult_y=2, zc=8)
+    o = Oracle(nx, ny, nz, p)
+    f = np.random.default_rng(2).standard_normal((nz, ny, nx))
+    for op in ("d1bydx", "d1bydy", "forwardfft", "inversefft", "dealias"):
+        assert rel(getattr(s, op)(f), getattr(o, op)(f)) < 1e-12, op
+    for op in ("d2bydx", "d2bydy"):
+        g1, g2 = getattr(s, op)(f)
+        w1, w2 = getattr(o, op)(f)
+        assert rel(g1, w1) < 1e-12 and rel(g2, w2) < 1e-12, op
+    s.close()
+
+
+@pytest.mark.parametrize("prec,tol,vy", [(0, 1e-10, 2), (1, 1e-5, 2), (1, 1e-5, [1, 2, 0, 2])])
+def test_second_alternative_plan_steps_match_oracle(prec, tol, vy):
+    # vy: one plan for every transform kind along y, or one per kind (d1, d1+d2, r2hc, hc2r)
+    from fug30v0kxc_amd import Solver
+    print(f"[autotune] plans {vy}, steps 256x256x12 precision {prec}", flush=True)
+    p = reference_params(t_amp=1e-3, b_amp=1e-3)
+    o = Oracle(256, 256, 12, p)
+    o.setup_static()
+    o.step(4, 1e-4, 0)
+    s = Solver(256, 256, 12, p, precision=prec)
+    s.set_tune(variant_x=2, variant_y=vy, mult_x=1, mult_y=1, zc=8)
+    s.static()
+    s.step(4, 1e-4, 0)
+    st = s.get_state()
+    bscale = max(np.abs(o[n]).max() for n in ("bx", "by", "bz"))
+    for n in NAMES:
+        scale = bscale if n.startswith("b") else np.abs(o[n]).max()
+        assert np.abs(st[n] - o[n]).max() / scale < tol, n
+    s.close()
+
+
 def test_set_tune_rejects_invalid_choices():
     from fug30v0kxc_amd import MhdError, Solver
     s = Solver(48, 48, 8, reference_params())         # 48 has no alternative plan
@@ -106,4 +148,11 @@ def test_set_tune_rejects_invalid_choices():
         s.set_tune(variant_x=1)
     with pytest.raises(MhdError):
         s.set_tune(mult_x=0)
+    s.close()
+    s = Solver(128, 256, 8, reference_params())       # 128: one alternative; 256: two
+    s.set_tune(variant_x=1, variant_y=2)
+    with pytest.raises(MhdError):
+        s.set_tune(variant_x=2)
+    with pytest.raises(MhdError):
+        s.set_tune(variant_y=3)
     s.close()
