@@ -151,24 +151,33 @@ def c4_specs(seed=C4_SEED, nbox=32):
 
 
 def ensemble_bench(steps, seed=C4_SEED):
-    """32 seeded independent boxes (c4_specs), mixed precision, each autotuned;
-    `steps` steps of all boxes by hipGraph replays against the same boxes
-    stepped one after another."""
+    """32 seeded independent boxes (c4_specs), mixed precision, autotuned once
+    per horizontal size (on the first box of that size; boxes of one size then
+    launch the same kernels, which the ensemble batches); `steps` steps of all
+    boxes by hipGraph replays against the same boxes stepped one after
+    another."""
     import numpy as np
     from fug30v0kxc_amd import MHD_PREC_MIXED, Ensemble, Solver
     specs = c4_specs(seed)
     dt = C4_DT
     cells = sum(n * n * nz for n, nz, _ in specs)
 
+    def shape(t):
+        return {k: v for k, v in t.items() if not k.startswith("ms_")}
+
     def make(tunes=None):
-        out = []
+        out, per_size = [], {}
         for i, (n, nz, p) in enumerate(specs):
             s = Solver(n, n, nz, p, precision=MHD_PREC_MIXED)
             s.static()
             if tunes is None:
-                s.tune = s.autotune()
+                if n not in per_size:
+                    per_size[n] = s.autotune()
+                else:
+                    s.set_tune(**shape(per_size[n]))
+                s.tune = per_size[n]
             else:
-                s.set_tune(**{k: v for k, v in tunes[i].items() if not k.startswith("ms_")})
+                s.set_tune(**shape(tunes[i]))
             out.append(s)
         return out
     ens_boxes = make()
@@ -467,10 +476,11 @@ def measured_traffic(precision, kernel):
 #           derivatives and history, plus the FP64 state read and written and
 #           the new FP32 shadows: 4 (5 + 4 + 26 + 10 + 5 + 5) + 8 (5 + 5) = 300 B.
 KERNEL_BYTES_PER_CELL = {"xderiv2": {8: 96, 4: 48}, "yderiv2": {8: 96, 4: 48}, "hydro": {8: 440, 4: 300}}
-KERNEL_DESC = {"xderiv2": "spec_kernel xderiv2 (d/dx and d2/dx2 of T, u, v, w)",
-               "yderiv2": "spec_kernel yderiv2 (d/dy and d2/dy2 of T, u, v, w)",
-               "hydro": "hydro_col_kernel (interior + wall rows): hydro_deriv's right-hand side after the "
-                        "x/y passes with the fused AB3 update of u, v, w, rho, T"}
+KERNEL_DESC = {"xderiv2": "SpecK<T, plan, 0, 2, 1, 0> (xderiv2: d/dx and d2/dx2 of T, u, v, w)",
+               "yderiv2": "SpecK<T, plan, 1, B, 1, 0> (yderiv2: d/dy and d2/dy2 of T, u, v, w)",
+               "hydro": "HydroColKernel<T, mixed, false> + <T, mixed, true> (hydro_col, interior and wall "
+                        "rows): hydro_deriv's right-hand side after the x/y passes with the fused AB3 "
+                        "update of u, v, w, rho, T"}
 
 
 # algorithmic bytes per cell-step of the whole step: 139 working-precision array
