@@ -132,43 +132,76 @@ struct TargetArgs {
     int zmax;
 };
 
+// One coarse cell's filtered fields and fluxes.  Each of the f lanes of a
+// cell (LANES = true: lane ii of the cell's f consecutive lanes, f a power of
+// two up to 64) sums fine column f I + ii over the cell's f rows and z window,
+// so a wave's loads cover 64 consecutive fine columns; the lanes' sums are then
+// added by xor butterflies within the cell's lane group.  LANES = false: one
+// thread per cell walks all f x f columns (any f).
+template <bool LANES>
 __global__ void __launch_bounds__(256) targets_kernel(TargetArgs a)
 {
-    const long q = (long)blockIdx.x * 256 + threadIdx.x;
-    if (q >= a.nc) return;
-    const long pc = (long)a.cx * a.cy;
-    const int K = (int)(q / pc);
-    const long rem = q - (long)K * pc;
-    const int J = (int)(rem / a.cx), I = (int)(rem - (long)J * a.cx);
+    const long g = (long)blockIdx.x * 256 + threadIdx.x;
     const int f = a.f;
+    const long q = LANES ? g / f : g;
+    const int lane = LANES ? (int)(g - q * f) : 0;
+    const bool live = q < a.nc;
+    if (!LANES && !live) return;
     const long nxy = (long)a.nx * a.ny;
     double s[CL_NF] = {0, 0, 0, 0, 0, 0, 0, 0};
     double uu[6] = {0, 0, 0, 0, 0, 0}, bb[6] = {0, 0, 0, 0, 0, 0}, uxb[3] = {0, 0, 0}, ut[3] = {0, 0, 0};
-    const int nk = a.znk[K];
-    for (int kk = 0; kk < nk; ++kk) {
-        const int k = a.zk0[K] + kk;
-        const double wz = a.zw[K * a.zmax + kk];
-        for (int jj = 0; jj < f; ++jj)
-            for (int ii = 0; ii < f; ++ii) {
-                const long p = (long)k * nxy + (long)(f * J + jj) * a.nx + (long)(f * I + ii);
-                double v[CL_NF];
+    int K = 0, J = 0, I = 0;
+    if (live) {
+        const long pc = (long)a.cx * a.cy;
+        K = (int)(q / pc);
+        const long rem = q - (long)K * pc;
+        J = (int)(rem / a.cx);
+        I = (int)(rem - (long)J * a.cx);
+        const int nk = a.znk[K];
+        const int i0 = LANES ? lane : 0, i1 = LANES ? lane + 1 : f;
+        for (int kk = 0; kk < nk; ++kk) {
+            const int k = a.zk0[K] + kk;
+            const double wz = a.zw[K * a.zmax + kk];
+            for (int jj = 0; jj < f; ++jj)
+                for (int ii = i0; ii < i1; ++ii) {
+                    const long p = (long)k * nxy + (long)(f * J + jj) * a.nx + (long)(f * I + ii);
+                    double v[CL_NF];
 #pragma unroll
-                for (int n = 0; n < CL_NF; ++n) {
-                    double x = a.src.f32[n] ? (double)((const float*)a.src.p[n])[p] : ((const double*)a.src.p[n])[p];
-                    if (a.src.mean[n]) x += a.src.mean[n][k];
-                    v[n] = x + a.src.add[n];
-                    s[n] += wz * v[n];
+                    for (int n = 0; n < CL_NF; ++n) {
+                        double x = a.src.f32[n] ? (double)((const float*)a.src.p[n])[p] : ((const double*)a.src.p[n])[p];
+                        if (a.src.mean[n]) x += a.src.mean[n][k];
+                        v[n] = x + a.src.add[n];
+                        s[n] += wz * v[n];
+                    }
+                    const double t = v[1], u = v[2], w2 = v[3], w = v[4], bx = v[5], by = v[6], bz = v[7];
+                    uu[0] += wz * (u * u); uu[1] += wz * (w2 * w2); uu[2] += wz * (w * w);
+                    uu[3] += wz * (u * w2); uu[4] += wz * (u * w); uu[5] += wz * (w2 * w);
+                    bb[0] += wz * (bx * bx); bb[1] += wz * (by * by); bb[2] += wz * (bz * bz);
+                    bb[3] += wz * (bx * by); bb[4] += wz * (bx * bz); bb[5] += wz * (by * bz);
+                    uxb[0] += wz * (w2 * bz - w * by);
+                    uxb[1] += wz * (w * bx - u * bz);
+                    uxb[2] += wz * (u * by - w2 * bx);
+                    ut[0] += wz * (u * t); ut[1] += wz * (w2 * t); ut[2] += wz * (w * t);
                 }
-                const double t = v[1], u = v[2], w2 = v[3], w = v[4], bx = v[5], by = v[6], bz = v[7];
-                uu[0] += wz * (u * u); uu[1] += wz * (w2 * w2); uu[2] += wz * (w * w);
-                uu[3] += wz * (u * w2); uu[4] += wz * (u * w); uu[5] += wz * (w2 * w);
-                bb[0] += wz * (bx * bx); bb[1] += wz * (by * by); bb[2] += wz * (bz * bz);
-                bb[3] += wz * (bx * by); bb[4] += wz * (bx * bz); bb[5] += wz * (by * bz);
-                uxb[0] += wz * (w2 * bz - w * by);
-                uxb[1] += wz * (w * bx - u * bz);
-                uxb[2] += wz * (u * by - w2 * bx);
-                ut[0] += wz * (u * t); ut[1] += wz * (w2 * t); ut[2] += wz * (w * t);
+        }
+    }
+    if constexpr (LANES) {
+        // every lane takes part in the butterflies; groups of f lanes are aligned
+        for (int o = 1; o < f; o <<= 1) {
+#pragma unroll
+            for (int n = 0; n < CL_NF; ++n) s[n] += __shfl_xor(s[n], o);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                uu[c] += __shfl_xor(uu[c], o);
+                bb[c] += __shfl_xor(bb[c], o);
             }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                uxb[c] += __shfl_xor(uxb[c], o);
+                ut[c] += __shfl_xor(ut[c], o);
+            }
+        }
+        if (!live || lane != 0) return;
     }
     const double inv = 1.0 / ((double)f * f);
     double m[CL_NF];
@@ -955,7 +988,10 @@ int launch_targets(const ClSource& src, int nx, int ny, int nz, int f, double fm
     a.zk0 = (const int*)((const unsigned char*)*zmem + w.size() * sizeof(double));
     a.znk = a.zk0 + cz;
     a.zmax = zmax;
-    hipLaunchKernelGGL(targets_kernel, dim3(nblk(a.nc)), dim3(256), 0, s, a);
+    if (f <= 64 && 64 % f == 0)
+        hipLaunchKernelGGL(targets_kernel<true>, dim3(nblk(a.nc * f)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(targets_kernel<false>, dim3(nblk(a.nc)), dim3(256), 0, s, a);
     return ok(hipGetLastError()) ? 0 : MHD_E_DEVICE;
 }
 

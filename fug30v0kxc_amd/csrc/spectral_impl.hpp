@@ -669,11 +669,12 @@ static int launch_plan(const SpecLaunch& L, hipStream_t s)
 {
     if (L.dir == 0) return launch_mode<TC, PL, 0, spec_b<TC, PL, 0>()>(L, s);
     constexpr int B0 = spec_b<TC, PL, 1>();
-    // FP32 only: at 256 points 8 pairs beat 16 for these passes (76, 72, 45 us
+    // FP32 only, where a full group would be a 1024-thread workgroup (one per
+    // CU): at 256 points 8 pairs beat 16 for these passes (76, 72, 45 us
     // against 81, 83, 48); in FP64 4 pairs lose to 8.  The other FP32 modes keep
     // 16 pairs (8 pairs: d1+d2 155 against 137 us, hc2r 125 against 109, r2hc
     // 63 against 58)
-    constexpr int BH = (sizeof(TC) == 4 && B0 >= 8) ? B0 / 2 : B0;
+    constexpr int BH = (sizeof(TC) == 4 && B0 >= 8 && B0 * plan_tpt<PL>() >= 1024) ? B0 / 2 : B0;
     if (BH != B0 && L.mode == SM_DERIV1 && L.g.nx % (2 * BH) == 0) return launch_y_deriv1<TC, PL, BH>(L, s);
     if (L.g.nx % (2 * B0) == 0) return launch_mode<TC, PL, 1, B0>(L, s);
     return launch_mode<TC, PL, 1, 1>(L, s);

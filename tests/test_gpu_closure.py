@@ -28,7 +28,11 @@ def _params(seed, scale=0.3):
     return (rng.standard_normal(po.N_PARAM) * scale).astype(np.float32)
 
 
-@pytest.mark.parametrize("f,shape", [(2, (8, 12, 16)), (4, (8, 16, 8))])
+# f a power of two: the lanes of a coarse cell split its fine columns (rows of
+# 8 to 256 fine points, so a wave spans one or several coarse rows); f = 3: one
+# thread per coarse cell
+@pytest.mark.parametrize("f,shape", [(2, (8, 12, 16)), (4, (8, 16, 8)), (8, (16, 16, 64)),
+                                     (4, (8, 8, 256)), (3, (9, 12, 24))])
 def test_targets_match_oracle(f, shape):
     import fug30v0kxc_amd as fa
     print(f"[closure] targets f={f} {shape}", flush=True)
