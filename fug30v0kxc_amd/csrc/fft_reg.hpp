@@ -10,8 +10,8 @@
 // exactly those registers -- need no LDS round trip between them, and the
 // inverse's last pass again holds x[j + r*N/R_0], ready to store.
 //
-// LDS indices are padded by one complex per 256 bytes (pad()), which keeps
-// the stride-R writes of the passes on distinct banks.
+// LDS indices are swizzled (pad()), which keeps the stride-R writes of the
+// passes on distinct banks.
 #pragma once
 
 #include "fft_lds.hpp"
@@ -195,7 +195,63 @@ struct RegFFT {
     static constexpr int PADSH = sizeof(T) == 8 ? 4 : 5;   // one pad per 256 bytes
     static constexpr int SP = N + (N >> PADSH) + 1;        // LDS stride per transform
 
-    static __device__ __forceinline__ int pad(int i) { return i + (i >> PADSH); }
+    // LDS slot of element i of a transform.  Lengths that are multiples of 16
+    // XOR the low four index bits with 5 (i >> 4): the exchanges' strided writes
+    // (stride Ns = 1, 4, 16, ... of the radix-4 and -8 passes) then fall on
+    // distinct banks within a 16-lane group, and contiguous reads stay
+    // conflict-free; other lengths pad one element per 256 bytes.
+    static __device__ __forceinline__ int pad(int i)
+    {
+        if constexpr (N % 16 == 0) return i ^ (((i >> 4) * 5) & 15);
+        else return i + (i >> PADSH);
+    }
+
+    // Twiddles in LDS, one table per pass: pass p (of the forward plan, or of
+    // the reversed plan for the inverse) keeps W_N^(r kk N/(Ns R)) for its
+    // butterflies' kk = j mod Ns at entry kk (R-1) + r-1 of its table.  The
+    // lanes of a wave then read entries R-1 apart; read from one table of
+    // W_N^i they were r N/(Ns R) apart, on few LDS banks.  A plan equal to its
+    // reverse shares the tables of both directions.
+    static constexpr int tw_count(int p, bool inv)
+    {
+        return plan_ns<PL>(p, inv) > 1 ? plan_ns<PL>(p, inv) * (plan_radix<PL>(p, inv) - 1) : 0;
+    }
+    static constexpr bool palindrome()
+    {
+        for (int p = 0; p < P; ++p)
+            if (plan_radix<PL>(p, false) != plan_radix<PL>(p, true)) return false;
+        return true;
+    }
+    static constexpr int tw_off(int p, bool inv)
+    {
+        int o = 0;
+        if (inv && !palindrome())
+            for (int q = 0; q < P; ++q) o += tw_count(q, false);
+        for (int q = 0; q < p; ++q) o += tw_count(q, inv);
+        return o;
+    }
+    static constexpr int TWN = tw_off(P, !palindrome());   // table entries in all
+    template <bool INV, int p>
+    static __device__ __forceinline__ void stage_pass(cpx<T>* tw, const cpx<T>* __restrict__ twg, int tid, int nt)
+    {
+        constexpr int R = plan_radix<PL>(p, INV);
+        constexpr int Ns = plan_ns<PL>(p, INV);
+        if constexpr (Ns > 1) {
+            constexpr int off = tw_off(p, INV), cnt = Ns * (R - 1), st = N / (Ns * R);
+            for (int e = tid; e < cnt; e += nt) {
+                const int kk = e / (R - 1), r = e - kk * (R - 1) + 1;
+                tw[off + e] = twg[r * kk * st];
+            }
+        }
+        if constexpr (p + 1 < P) stage_pass<INV, p + 1>(tw, twg, tid, nt);
+    }
+    // fill the tables (TWN entries) from W_N^i (i < N) in global memory
+    static __device__ __forceinline__ void stage_twiddles(cpx<T>* tw, const cpx<T>* __restrict__ twg, int tid,
+                                                          int nt)
+    {
+        stage_pass<false, 0>(tw, twg, tid, nt);
+        if constexpr (!palindrome()) stage_pass<true, 0>(tw, twg, tid, nt);
+    }
 
     // butterfly of pass p on v (thread's butterfly index j)
     template <int p, bool INV>
@@ -205,9 +261,10 @@ struct RegFFT {
         constexpr int Ns = plan_ns<PL>(p, INV);
         if constexpr (Ns > 1) {
             const int kk = j % Ns;
+            const cpx<T>* __restrict__ twp = tw + tw_off(p, INV) + kk * (R - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                const cpx<T> w = tw[r * kk * (N / (Ns * R))];
+                const cpx<T> w = twp[r - 1];
                 v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
             }
         }

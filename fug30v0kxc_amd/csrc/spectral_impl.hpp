@@ -295,7 +295,6 @@ template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
 __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_cross3(const SpecLaunch& L, uint3 bid,
                                                                               uint3 gdim)
 {
-    constexpr int N = PL::N;
     using F = RegFFT<TC, PL>;
     using LN = Lines<PL, DIR, B>;
     constexpr int NB0 = F::NB0, R0 = F::R0;
@@ -303,10 +302,10 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_cross3(co
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<TC>* twl = reinterpret_cast<cpx<TC>*>(smem);
     const cpx<TC>* __restrict__ twg = reinterpret_cast<const cpx<TC>*>(L.tw);
-    for (int i = threadIdx.x; i < N; i += NT) twl[i] = twg[i];
+    F::stage_twiddles(twl, twg, threadIdx.x, NT);
     int t, j;
     LN::lane(threadIdx.x, t, j);
-    cpx<TC>* lds = twl + N + t * F::SP;
+    cpx<TC>* lds = twl + F::TWN + t * F::SP;
     const long ngrp = LN::groups(g);
     const long G = gdim.x;
     // the factors: job q loads {a_{q+1}, b_{q+2}, a_{q+2}, b_{q+1}}
@@ -385,10 +384,10 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_jobs(cons
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<TC>* twl = reinterpret_cast<cpx<TC>*>(smem);
     const cpx<TC>* __restrict__ twg = reinterpret_cast<const cpx<TC>*>(L.tw);
-    for (int i = threadIdx.x; i < N; i += NT) twl[i] = twg[i];
+    F::stage_twiddles(twl, twg, threadIdx.x, NT);
     int t, j;
     LN::lane(threadIdx.x, t, j);
-    cpx<TC>* lds = twl + N + t * F::SP;
+    cpx<TC>* lds = twl + F::TWN + t * F::SP;
     const long ngrp = LN::groups(g);
     const long G = gdim.x;
     const bool gmaj = L.group_major != 0;
@@ -576,7 +575,7 @@ constexpr int spec_b()
 {
     constexpr int tpt = plan_tpt<PL>();
     constexpr int sp = RegFFT<TC, PL>::SP * (int)sizeof(cpx<TC>);
-    constexpr int twb = PL::N * (int)sizeof(cpx<TC>);
+    constexpr int twb = RegFFT<TC, PL>::TWN * (int)sizeof(cpx<TC>);
     constexpr int lim = (DIR == 1 && sizeof(TC) == 8) ? SPEC_LDS_Y : SPEC_LDS_X;
     int b = DIR == 0 ? (128 / tpt > 0 ? 128 / tpt : 1) : (sizeof(TC) == 8 ? 8 : 16);
     while (b > 1 && (b * tpt > 1024 || twb + b * sp > lim)) b /= 2;
@@ -586,7 +585,7 @@ constexpr int spec_b()
 template <typename TC, typename PL, int B>
 constexpr size_t spec_lds_bytes()
 {
-    return (size_t)(PL::N + B * RegFFT<TC, PL>::SP) * sizeof(cpx<TC>);
+    return (size_t)(RegFFT<TC, PL>::TWN + B * RegFFT<TC, PL>::SP) * sizeof(cpx<TC>);
 }
 
 template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
