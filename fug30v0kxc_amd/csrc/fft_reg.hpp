@@ -193,7 +193,8 @@ struct RegFFT {
     static constexpr int NB0 = N / R0;   // butterflies of the first forward pass
     static constexpr int NBL = N / RL;   // butterflies of the last forward pass
     static constexpr int PADSH = sizeof(T) == 8 ? 4 : 5;   // one pad per 256 bytes
-    static constexpr int SP = N + (N >> PADSH) + 1;        // LDS stride per transform
+    // LDS stride per transform
+    static constexpr int SP = (N % 16 == 0) ? N + 6 : N + (N >> PADSH) + 1;
 
     // LDS slot of element i of a transform.  Lengths that are multiples of 16
     // XOR the low four index bits with 5 (i >> 4): the exchanges' strided writes
