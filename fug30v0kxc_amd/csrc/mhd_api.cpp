@@ -1787,6 +1787,12 @@ int mhd_ensemble_block(void) { return ENS_BLOCK; }
 }  // extern "C"
 namespace {
 // average milliseconds of `reps` calls of f on the context's stream
+// Autotuner timing: repetitions per candidate, and the margin a later
+// candidate must win by to replace an earlier one (the defaults come first),
+// so run-to-run noise of about 1 % does not move the choice.
+constexpr int TUNE_REPS = 5;
+constexpr double TUNE_GAIN = 0.99;
+
 template <typename F>
 int time_ms(mhd_ctx* c, int reps, F&& f, double* ms)
 {
@@ -1872,8 +1878,8 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
             c->spec_mult[dir] = mult;
             const SpecLaunch L = kind_launch(dir, SM_DERIV2);
             double ms = 0.0;
-            TRY(time_ms(c, 3, [&] { return spec(c, L); }, &ms));
-            if (ms < best) { best = ms; bm = mult; }
+            TRY(time_ms(c, TUNE_REPS, [&] { return spec(c, L); }, &ms));
+            if (ms < TUNE_GAIN * best) { best = ms; bm = mult; }
         }
         c->spec_mult[dir] = bm;
         for (int kind = 0; kind < MHD_TK_N; ++kind) {
@@ -1883,8 +1889,8 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
                 c->spec_var[dir][kind] = var;
                 const SpecLaunch L = kind_launch(dir, kind);
                 double ms = 0.0;
-                TRY(time_ms(c, 3, [&] { return spec(c, L); }, &ms));
-                if (ms < bk) { bk = ms; bv = var; }
+                TRY(time_ms(c, TUNE_REPS, [&] { return spec(c, L); }, &ms));
+                if (ms < TUNE_GAIN * bk) { bk = ms; bv = var; }
             }
             c->spec_var[dir][kind] = bv;
             if (kind == SM_DERIV2) best_ms[dir] = bk;
@@ -1907,8 +1913,8 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
             fuse_ab3(c, a, cf0, W[W_RX], W[W_RX]);
         }
         double ms = 0.0;
-        TRY(time_ms(c, 3, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); }, &ms));
-        if (ms < bestc) { bestc = ms; bz = zc; }
+        TRY(time_ms(c, TUNE_REPS, [&] { return kerr(launch_hydro_col(a, c->wf32, c->s)); }, &ms));
+        if (ms < TUNE_GAIN * bestc) { bestc = ms; bz = zc; }
     }
     c->hzc = bz;
     c->timed_kernel = saved_timing;
