@@ -182,7 +182,10 @@ __device__ __forceinline__ void dft_r(cpx<T>* v)
     else butterfly<T, R, INV>(v);
 }
 
-template <typename T, typename PL>
+// WAVE: every transform's butterflies lie within one wavefront (and each
+// transform has its own LDS area), so the exchanges only need the wave's own
+// LDS operations ordered, not a workgroup barrier.
+template <typename T, typename PL, bool WAVE = false>
 struct RegFFT {
     static constexpr int N = PL::N;
     static constexpr int P = PL::P;
@@ -201,6 +204,19 @@ struct RegFFT {
     // (stride Ns = 1, 4, 16, ... of the radix-4 and -8 passes) then fall on
     // distinct banks within a 16-lane group, and contiguous reads stay
     // conflict-free; other lengths pad one element per 256 bytes.
+    // orders the LDS accesses around an exchange: a workgroup barrier, or, for
+    // wave-local transforms, a compiler fence (a wave's LDS operations are
+    // performed in order)
+    static __device__ __forceinline__ void sync()
+    {
+        if constexpr (WAVE) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    }
     static __device__ __forceinline__ int pad(int i)
     {
         if constexpr (N % 16 == 0) return i ^ (((i >> 4) * 5) & 15);
@@ -284,14 +300,14 @@ struct RegFFT {
 #pragma unroll
             for (int r = 0; r < R; ++r) lds[pad(base + r * Ns)] = v[r];
         }
-        __syncthreads();
+        sync();
         constexpr int R2 = plan_radix<PL>(p + 1, INV);
         constexpr int NB2 = N / R2;
         if (j < NB2) {
 #pragma unroll
             for (int r = 0; r < R2; ++r) v[r] = lds[pad(j + r * NB2)];
         }
-        __syncthreads();
+        sync();
     }
 
     template <int p, bool INV>

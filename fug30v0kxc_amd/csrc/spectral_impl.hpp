@@ -245,6 +245,11 @@ template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
 struct SpecK {
     using Args = SpecLaunch;
     static constexpr int NT = B * plan_tpt<PL>();
+    // each transform within one wavefront: rows with at most 64 butterflies
+    // per transform, or a workgroup of one wave
+    static constexpr bool WAVE_LOCAL =
+        NT <= 64 || (DIR == 0 && plan_tpt<PL>() <= 64 && 64 % plan_tpt<PL>() == 0);
+    using F = RegFFT<TC, PL, WAVE_LOCAL>;
     static __device__ __forceinline__ void body(const SpecLaunch& L, uint3 bid, uint3 gdim)
     {
         if constexpr (LOP == LD_CROSS3) body_cross3(L, bid, gdim);
@@ -263,14 +268,13 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::r2hc_store(cpx
                                                                              long st)
 {
     constexpr int N = PL::N;
-    using F = RegFFT<TC, PL>;
     constexpr int NB0 = F::NB0, NBL = F::NBL, R0 = F::R0, RL = F::RL;
     // spectrum -> LDS, then separate the pair into two halfcomplex lines
     if (j < NBL) {
 #pragma unroll
         for (int r = 0; r < RL; ++r) lds[F::pad(j + r * NBL)] = v[r];
     }
-    __syncthreads();
+    F::sync();
     if (ok && j < NB0) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
@@ -288,14 +292,13 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::r2hc_store(cpx
             }
         }
     }
-    __syncthreads();
+    F::sync();
 }
 
 template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
 __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_cross3(const SpecLaunch& L, uint3 bid,
                                                                               uint3 gdim)
 {
-    using F = RegFFT<TC, PL>;
     using LN = Lines<PL, DIR, B>;
     constexpr int NB0 = F::NB0, R0 = F::R0;
     const Geom g = L.g;
@@ -377,7 +380,6 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_jobs(cons
                                                                             uint3 gdim)
 {
     constexpr int N = PL::N;
-    using F = RegFFT<TC, PL>;
     using LN = Lines<PL, DIR, B>;
     constexpr int NB0 = F::NB0, NBL = F::NBL, R0 = F::R0, RL = F::RL;
     const Geom g = L.g;
@@ -494,7 +496,7 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_jobs(cons
 #pragma unroll
                 for (int r = 0; r < R0; ++r) lds[F::pad(j + r * NB0)] = v[r];
             }
-            __syncthreads();
+            F::sync();
             if (j < NBL) {
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
@@ -508,7 +510,7 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_jobs(cons
                     v[r] = {A.x - Bv.y, A.y + Bv.x};
                 }
             }
-            __syncthreads();
+            F::sync();
             F::inverse(v, j, lds, twl);
             if (ok && j < NB0) {
 #pragma unroll
