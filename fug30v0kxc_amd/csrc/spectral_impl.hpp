@@ -579,7 +579,9 @@ constexpr int spec_b()
     constexpr int sp = RegFFT<TC, PL>::SP * (int)sizeof(cpx<TC>);
     constexpr int twb = RegFFT<TC, PL>::TWN * (int)sizeof(cpx<TC>);
     constexpr int lim = (DIR == 1 && sizeof(TC) == 8) ? SPEC_LDS_Y : SPEC_LDS_X;
-    int b = DIR == 0 ? (128 / tpt > 0 ? 128 / tpt : 1) : (sizeof(TC) == 8 ? 8 : 16);
+    // FP64 y groups of 8 pairs (128-byte rows); 16 (256-byte rows) where a
+    // transform takes at most 16 threads, so the group stays at 256 threads
+    int b = DIR == 0 ? (128 / tpt > 0 ? 128 / tpt : 1) : (sizeof(TC) == 8 ? (tpt <= 16 ? 16 : 8) : 16);
     while (b > 1 && (b * tpt > 1024 || twb + b * sp > lim)) b /= 2;
     return b;
 }
