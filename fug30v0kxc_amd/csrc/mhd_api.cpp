@@ -1861,10 +1861,11 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
             else L.job[j] = sjob(in[j], sf, a, nullptr, 0, 0, wf);
         }
         if (kind == SM_DERIV1) {
-            // the step's first derivatives mix plain, product and Maxwell-stress
-            // loads in one launch: so does the timed one
-            L.job[1].load_op = LD_PROD; L.job[1].in[1] = in[2];
-            L.job[2].load_op = LD_MAXW; L.job[2].in[1] = in[3]; L.job[2].in[2] = in[0];
+            // as in the step: a plain first derivative beside the three of one
+            // direction's Maxwell-stress loads (products of three fields)
+            L.job[1].in[0] = in[2]; L.job[1].load_op = LD_PROD; L.job[1].in[1] = in[1];
+            L.job[2].in[0] = in[3]; L.job[2].load_op = LD_PROD; L.job[2].in[1] = in[1];
+            L.job[3].in[0] = in[1]; L.job[3].load_op = LD_MAXW; L.job[3].in[1] = in[2]; L.job[3].in[2] = in[3];
         }
         L.njobs = 4;
         return L;

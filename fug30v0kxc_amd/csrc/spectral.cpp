@@ -75,13 +75,51 @@ static bool is_cross_product(const SpecLaunch& L)
     return true;
 }
 
+// Among first-derivative jobs, three that differentiate the Maxwell-stress
+// loads of one direction: MAXW(b0, b1, b2), PROD(b1, b0) or PROD(b0, b1), and
+// PROD(b2, b0), all of one element type.  Fills G with those three (in that
+// order, job 0 the first product) and R with the others; false if absent.
+static bool split_bstress(const SpecLaunch& L, SpecLaunch& G, SpecLaunch& R)
+{
+    if (L.mode != SM_DERIV1 || L.njobs < 3) return false;
+    for (int m = 0; m < L.njobs; ++m) {
+        const SpecJob& M = L.job[m];
+        if (M.load_op != LD_MAXW) continue;
+        const void *b0 = M.in[0], *b1 = M.in[1], *b2 = M.in[2];
+        int p1 = -1, p2 = -1, pat = 0;
+        for (int q = 0; q < L.njobs; ++q) {
+            const SpecJob& J = L.job[q];
+            if (q == m || J.load_op != LD_PROD || J.in_f32 != M.in_f32) continue;
+            if (p1 < 0 && J.in[0] == b1 && J.in[1] == b0) { p1 = q; pat = 0; }
+            else if (p1 < 0 && J.in[0] == b0 && J.in[1] == b1) { p1 = q; pat = 1; }
+            else if (p2 < 0 && J.in[0] == b2 && J.in[1] == b0) p2 = q;
+        }
+        if (p1 < 0 || p2 < 0) continue;
+        G = L;
+        G.njobs = 3;
+        G.job[0] = L.job[p1];
+        G.job[1] = L.job[p2];
+        G.job[2] = M;
+        G.bstress = 1;
+        G.bpat = pat;
+        R = L;
+        R.njobs = 0;
+        for (int q = 0; q < L.njobs; ++q)
+            if (q != m && q != p1 && q != p2) R.job[R.njobs++] = L.job[q];
+        return true;
+    }
+    return false;
+}
+
 // The load operation is a compile-time parameter of the kernels, so jobs are
-// launched in groups of equal load operation -- except first-derivative jobs
-// of the plain, product and Maxwell loads on the register-blocked lengths,
-// which share one launch of the run-time-dispatch kernel (LD_ANY): one
-// larger launch streams at a higher rate than three small ones; and the
-// three components of a cross product along x on those lengths, which share
-// their six factor loads (LD_CROSS3).
+// launched in groups of equal load operation -- except, on the
+// register-blocked lengths: the three first derivatives of one direction's
+// Maxwell-stress loads, which share one load of the field components
+// (LD_BSTRESS); other first-derivative jobs of the plain, product and
+// Maxwell loads, which share one launch of the run-time-dispatch kernel
+// (LD_ANY: one larger launch streams at a higher rate than three small ones);
+// and the three components of a cross product along x, which share their six
+// factor loads (LD_CROSS3).
 int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
 {
     if (L.njobs <= 0) return 0;
@@ -89,6 +127,17 @@ int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
         SpecLaunch G = L;
         G.cross3 = 1;
         return launch_group(G, compute_f32, s);
+    }
+    // not for FP64 columns: there the triple kernel (152 VGPRs) fits one
+    // 512-thread workgroup per CU and measured slower than the per-job loads
+    // (292 against 282 us at 256^2 x 160)
+    if (spectral_len_fixed(L.dir == 0 ? L.g.nx : L.g.ny) && (L.dir == 0 || compute_f32)) {
+        SpecLaunch G, R;
+        if (split_bstress(L, G, R)) {
+            const int e = launch_group(G, compute_f32, s);
+            if (e || R.njobs == 0) return e;
+            return launch_spectral(R, compute_f32, s);
+        }
     }
     if (L.mode == SM_DERIV1 && L.njobs > 1 && spectral_len_fixed(L.dir == 0 ? L.g.nx : L.g.ny)) {
         bool mergeable = true, mixed_ops = false;

@@ -29,7 +29,12 @@ enum LoadOp { LD_PLAIN = 0, LD_PROD = 1, LD_CROSS = 2, LD_BG = 3, LD_MAXW = 4,
               // components of one cross product a x b, job q loading
               // {a_{q+1}, b_{q+2}, a_{q+2}, b_{q+1}} (indices mod 3); the six
               // factor lines of a line group are loaded once for all three
-              LD_CROSS3 = 6 };
+              LD_CROSS3 = 6,
+              // kernel template value only: first derivatives of the three
+              // Maxwell-stress loads of one direction, PROD(b1, b0) or PROD(b0, b1),
+              // PROD(b2, b0) and MAXW(b0, b1, b2) (SpecLaunch::bpat), from one
+              // load of the three field components per line group
+              LD_BSTRESS = 7 };
 
 constexpr int SPEC_MAX_JOBS = 8;
 // lengths without a register-blocked plan: run-time radix plans up to this length
@@ -73,6 +78,8 @@ struct SpecLaunch {
     int group_major;  // set by the launcher: items ordered line group first (see spec_kernel)
     int any_load;     // set by the launcher: jobs of mixed load operations in one LD_ANY launch
     int cross3;       // set by the launcher: the three jobs form one cross product (LD_CROSS3)
+    int bstress;      // set by the launcher: the three jobs are Maxwell-stress loads (LD_BSTRESS)
+    int bpat;         // ... job 0 is PROD(b1, b0) (0) or PROD(b0, b1) (1)
     // set by the launcher for lengths without a register-blocked plan: the
     // run-time radices and the complex lines per workgroup
     int gen_radix[SPEC_GEN_MAX_PASSES];

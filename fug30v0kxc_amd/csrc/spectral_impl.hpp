@@ -124,6 +124,24 @@ struct Loader {
             return cross_pair(ld_pair(a, i), ld_pair(b, i), ld_pair(c, i), ld_pair(d, i), i);
         }
     }
+    // PLAIN, PROD or MAXW from factor values already loaded (x = in0, y = in1,
+    // z = in2; both elements of a pair), the arithmetic of pair()
+    __device__ __forceinline__ cpx<TC> combine(cpx<TC> x, cpx<TC> y, cpx<TC> z) const
+    {
+        if constexpr (LOP == LD_PROD) {
+            return {(scale * x.x) * y.x + scale * (x.x * m1 + m0 * y.x),
+                    (scale * x.y) * y.y + scale * (x.y * m1 + m0 * y.y)};
+        } else if constexpr (LOP == LD_MAXW) {
+            return {scale * x.x * x.x - scale * y.x * y.x - scale * z.x * z.x +
+                        (TC)2 * scale * (m0 * x.x - m1 * y.x - m2 * z.x),
+                    scale * x.y * x.y - scale * y.y * y.y - scale * z.y * z.y +
+                        (TC)2 * scale * (m0 * x.y - m1 * y.y - m2 * z.y)};
+        } else {
+            (void)y;
+            (void)z;
+            return x;
+        }
+    }
     // LD_CROSS from factor values already loaded (x = in0, y = in1, z = in2,
     // w = in3 at element i), the extra term loaded here
     __device__ __forceinline__ TC cross(TC x, TC y, TC z, TC w, long i) const
@@ -253,8 +271,10 @@ struct SpecK {
     static __device__ __forceinline__ void body(const SpecLaunch& L, uint3 bid, uint3 gdim)
     {
         if constexpr (LOP == LD_CROSS3) body_cross3(L, bid, gdim);
+        else if constexpr (LOP == LD_BSTRESS) body_bstress(L, bid, gdim);
         else body_jobs(L, bid, gdim);
     }
+    static __device__ __forceinline__ void body_bstress(const SpecLaunch& L, uint3 bid, uint3 gdim);
     static __device__ __forceinline__ void body_jobs(const SpecLaunch& L, uint3 bid, uint3 gdim);
     static __device__ __forceinline__ void body_cross3(const SpecLaunch& L, uint3 bid, uint3 gdim);
     // forward spectrum in v -> the two halfcomplex lines of the pair, stored
@@ -371,6 +391,109 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_cross3(co
             for (int r = 0; r < R0; ++r) v[r] = cur[q][r];
             F::forward(v, j, lds, twl);
             r2hc_store(v, j, lds, ok, (TC*)L.job[q].out[0], ia, ib, st);
+        }
+    }
+}
+
+// The first derivatives of the three Maxwell-stress loads of a direction: an
+// item is one line group; the three field components b0, b1, b2 (job 2's
+// inputs) are loaded once, each job's input formed from them with the job's
+// own means and scale (the arithmetic of the per-job loader), and the three
+// transforms run back to back.
+template <typename TC, typename PL, int DIR, int B, int MODE, int LOP>
+__device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_bstress(const SpecLaunch& L, uint3 bid,
+                                                                               uint3 gdim)
+{
+    constexpr int N = PL::N;
+    using LN = Lines<PL, DIR, B>;
+    constexpr int NB0 = F::NB0, NBL = F::NBL, R0 = F::R0, RL = F::RL;
+    const Geom g = L.g;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<TC>* twl = reinterpret_cast<cpx<TC>*>(smem);
+    const cpx<TC>* __restrict__ twg = reinterpret_cast<const cpx<TC>*>(L.tw);
+    F::stage_twiddles(twl, twg, threadIdx.x, NT);
+    int t, j;
+    LN::lane(threadIdx.x, t, j);
+    cpx<TC>* lds = twl + F::TWN + t * F::SP;
+    const long ngrp = LN::groups(g);
+    const long G = gdim.x;
+    const double cx = L.cx;
+    const int icut = L.icut;
+    const TC* __restrict__ f0 = (const TC*)L.job[2].in[0];
+    const TC* __restrict__ f1 = (const TC*)L.job[2].in[1];
+    const TC* __restrict__ f2 = (const TC*)L.job[2].in[2];
+    const bool pat = L.bpat != 0;   // job 0 is PROD(b0, b1) rather than PROD(b1, b0)
+    __syncthreads();
+
+    auto fetch = [&](long grp, cpx<TC> (*dst)[R0]) {
+        long ia, ib, st;
+        int k;
+        const bool ok = LN::base(g, grp, t, ia, ib, st, k);
+        const Loader<TC, LD_PROD> l0(L.job[0], ok ? k : 0), l1(L.job[1], ok ? k : 0);
+        const Loader<TC, LD_MAXW> l2(L.job[2], ok ? k : 0);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int n = j + r * NB0;
+            dst[0][r] = dst[1][r] = dst[2][r] = {(TC)0, (TC)0};
+            if (ok && j < NB0) {
+                cpx<TC> a, b, c;
+                if constexpr (DIR == 1) {
+                    const long i = ia + n * st;
+                    a = ld_pair(f0, i);
+                    b = ld_pair(f1, i);
+                    c = ld_pair(f2, i);
+                } else {
+                    const long i = ia + n * st, e = ib + n * st;
+                    a = {f0[i], f0[e]};
+                    b = {f1[i], f1[e]};
+                    c = {f2[i], f2[e]};
+                }
+                dst[0][r] = pat ? l0.combine(a, b, c) : l0.combine(b, a, c);
+                dst[1][r] = l1.combine(c, a, b);
+                dst[2][r] = l2.combine(a, b, c);
+            }
+        }
+    };
+
+    cpx<TC> nxt[3][R0];
+    long grp = bid.x;
+    if (grp < ngrp) fetch(grp, nxt);
+    for (; grp < ngrp; grp += G) {
+        cpx<TC> cur[3][R0];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int r = 0; r < R0; ++r) cur[q][r] = nxt[q][r];
+        long ia, ib, st;
+        int k;
+        const bool ok = LN::base(g, grp, t, ia, ib, st, k);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const SpecJob& J = L.job[q];
+            // the next group's loads fly during the last job's transform
+            if (q == 2 && grp + G < ngrp) fetch(grp + G, nxt);
+            cpx<TC> v[F::RMAX];
+#pragma unroll
+            for (int r = 0; r < R0; ++r) v[r] = cur[q][r];
+            F::forward(v, j, lds, twl);
+            if (j < NBL) {
+#pragma unroll
+                for (int r = 0; r < RL; ++r) v[r] = apply_mul<TC, N>(J.mul[0], v[r], j + r * NBL, cx, icut, J.mc);
+            }
+            F::inverse(v, j, lds, twl);
+            if (ok && j < NB0) {
+                TC* __restrict__ out = (TC*)J.out[0];
+                const TC add = (J.addk ? (TC)J.addk[k] : (TC)0) + (TC)J.adds;
+#pragma unroll
+                for (int r = 0; r < R0; ++r) {
+                    const int n = j + r * NB0;
+                    if constexpr (DIR == 1) st_pair_nt(out, ia + n * st, v[r].x + add, v[r].y + add);
+                    else {
+                        st_nt(out, ia + n * st, v[r].x + add);
+                        st_nt(out, ib + n * st, v[r].y + add);
+                    }
+                }
+            }
         }
     }
 }
@@ -600,8 +723,8 @@ static int launch_one(const SpecLaunch& L, hipStream_t s)
     constexpr size_t lds = spec_lds_bytes<TC, PL, B>();
     static_assert(lds <= (size_t)SPEC_LDS_Y, "transform workgroup exceeds 160 KB of LDS");
     auto kern = single_kernel<K>;
-    // LD_CROSS3: one item per line group for all three jobs
-    const long items = Lines<PL, DIR, B>::groups(L.g) * (LOP == LD_CROSS3 ? 1 : L.njobs);
+    // LD_CROSS3, LD_BSTRESS: one item per line group for all three jobs
+    const long items = Lines<PL, DIR, B>::groups(L.g) * (LOP == LD_CROSS3 || LOP == LD_BSTRESS ? 1 : L.njobs);
     // more than 64 KB of dynamic LDS is opted into once per kernel
     static int lds_ok = lds <= 65536 ? 1 : 0;
     if (!lds_ok) {
@@ -634,6 +757,7 @@ template <typename TC, typename PL, int DIR, int B>
 static int launch_mode(const SpecLaunch& L, hipStream_t s)
 {
     if (L.any_load) return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_ANY>(L, s);
+    if (L.bstress) return launch_one<TC, PL, DIR, B, SM_DERIV1, LD_BSTRESS>(L, s);
     if constexpr (DIR == 0) {
         if (L.cross3) return launch_one<TC, PL, DIR, B, SM_R2HC, LD_CROSS3>(L, s);
     }
@@ -659,6 +783,7 @@ template <typename TC, typename PL, int B>
 static int launch_y_deriv1(const SpecLaunch& L, hipStream_t s)
 {
     if (L.any_load) return launch_one<TC, PL, 1, B, SM_DERIV1, LD_ANY>(L, s);
+    if (L.bstress) return launch_one<TC, PL, 1, B, SM_DERIV1, LD_BSTRESS>(L, s);
     switch (L.job[0].load_op) {
     case LD_PLAIN: return launch_one<TC, PL, 1, B, SM_DERIV1, LD_PLAIN>(L, s);
     case LD_PROD: return launch_one<TC, PL, 1, B, SM_DERIV1, LD_PROD>(L, s);
