@@ -277,6 +277,22 @@ struct SpecK {
     static __device__ __forceinline__ void body_bstress(const SpecLaunch& L, uint3 bid, uint3 gdim);
     static __device__ __forceinline__ void body_jobs(const SpecLaunch& L, uint3 bid, uint3 gdim);
     static __device__ __forceinline__ void body_cross3(const SpecLaunch& L, uint3 bid, uint3 gdim);
+    // Column groups narrower than a 128-byte line (y, 2 B sizeof(TC) < 128
+    // bytes) share each line with the next group.  Within every 16 items, item
+    // c (< 8) takes group 2c and item c + 8 group 2c + 1: workgroups w and w + 8
+    // run on the same XCD (they are dealt round-robin over 8) at the same time,
+    // so a line is fetched once.  Identity otherwise.
+    static __device__ __forceinline__ long paired_group(long item, long G, long ngrp)
+    {
+        constexpr bool half_line = DIR == 1 && 2 * B * (int)sizeof(TC) < 128;
+        if constexpr (half_line) {
+            if (G % 16 == 0 && ngrp % 16 == 0) {
+                const long c = item & 15;
+                return item - c + (c < 8 ? 2 * c : 2 * (c - 8) + 1);
+            }
+        }
+        return item;
+    }
     // forward spectrum in v -> the two halfcomplex lines of the pair, stored
     static __device__ __forceinline__ void r2hc_store(cpx<TC>* v, int j, cpx<TC>* lds, bool ok, TC* __restrict__ out0,
                                                       long ia, long ib, long st);
@@ -371,9 +387,10 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_cross3(co
     };
 
     cpx<TC> nxt[3][R0];
-    long grp = bid.x;
-    if (grp < ngrp) fetch(grp, nxt);
-    for (; grp < ngrp; grp += G) {
+    long item = bid.x;
+    if (item < ngrp) fetch(paired_group(item, G, ngrp), nxt);
+    for (; item < ngrp; item += G) {
+        const long grp = paired_group(item, G, ngrp);
         cpx<TC> cur[3][R0];
 #pragma unroll
         for (int q = 0; q < 3; ++q)
@@ -385,7 +402,7 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_cross3(co
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             // the next group's loads fly during the last job's transform
-            if (q == 2 && grp + G < ngrp) fetch(grp + G, nxt);
+            if (q == 2 && item + G < ngrp) fetch(paired_group(item + G, G, ngrp), nxt);
             cpx<TC> v[F::RMAX];
 #pragma unroll
             for (int r = 0; r < R0; ++r) v[r] = cur[q][r];
@@ -456,9 +473,10 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_bstress(c
     };
 
     cpx<TC> nxt[3][R0];
-    long grp = bid.x;
-    if (grp < ngrp) fetch(grp, nxt);
-    for (; grp < ngrp; grp += G) {
+    long item = bid.x;
+    if (item < ngrp) fetch(paired_group(item, G, ngrp), nxt);
+    for (; item < ngrp; item += G) {
+        const long grp = paired_group(item, G, ngrp);
         cpx<TC> cur[3][R0];
 #pragma unroll
         for (int q = 0; q < 3; ++q)
@@ -471,7 +489,7 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_bstress(c
         for (int q = 0; q < 3; ++q) {
             const SpecJob& J = L.job[q];
             // the next group's loads fly during the last job's transform
-            if (q == 2 && grp + G < ngrp) fetch(grp + G, nxt);
+            if (q == 2 && item + G < ngrp) fetch(paired_group(item + G, G, ngrp), nxt);
             cpx<TC> v[F::RMAX];
 #pragma unroll
             for (int r = 0; r < R0; ++r) v[r] = cur[q][r];
