@@ -128,10 +128,10 @@ int launch_spectral(const SpecLaunch& L, int compute_f32, hipStream_t s)
         G.cross3 = 1;
         return launch_group(G, compute_f32, s);
     }
-    // not for FP64 columns: there the triple kernel (152 VGPRs) fits one
-    // 512-thread workgroup per CU and measured slower than the per-job loads
-    // (292 against 282 us at 256^2 x 160)
-    if (spectral_len_fixed(L.dir == 0 ? L.g.nx : L.g.ny) && (L.dir == 0 || compute_f32)) {
+    // FP64 columns too, at half column groups (see launch_plan): the triple and
+    // the remaining plain jobs took 274 us at 256^2 x 160 against 282 for
+    // per-job loads of all six
+    if (spectral_len_fixed(L.dir == 0 ? L.g.nx : L.g.ny)) {
         SpecLaunch G, R;
         if (split_bstress(L, G, R)) {
             const int e = launch_group(G, compute_f32, s);

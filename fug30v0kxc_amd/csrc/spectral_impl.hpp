@@ -822,6 +822,12 @@ static int launch_plan(const SpecLaunch& L, hipStream_t s)
     // 63 against 58)
     constexpr int BH = (sizeof(TC) == 4 && B0 >= 8 && B0 * plan_tpt<PL>() >= 1024) ? B0 / 2 : B0;
     if (BH != B0 && L.mode == SM_DERIV1 && L.g.nx % (2 * BH) == 0) return launch_y_deriv1<TC, PL, BH>(L, s);
+    // FP64 Maxwell-stress triple (about 150 VGPRs): half groups, 3 workgroups
+    // per CU instead of 1 (at 256^2 x 160: 153 against 171 us; the groups
+    // sharing a line are paired onto one XCD, so each line is fetched once)
+    if constexpr (sizeof(TC) == 8 && B0 >= 8) {
+        if (L.bstress && L.g.nx % B0 == 0) return launch_one<TC, PL, 1, B0 / 2, SM_DERIV1, LD_BSTRESS>(L, s);
+    }
     if (L.g.nx % (2 * B0) == 0) return launch_mode<TC, PL, 1, B0>(L, s);
     return launch_mode<TC, PL, 1, 1>(L, s);
 }
