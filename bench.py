@@ -19,8 +19,8 @@ hydro_col (the whole hydrodynamic right-hand side after the x/y transform
 passes, with the fused AB3 update of u, v, w, rho, T: 55 arrays per cell in
 FP64, 300 bytes per cell in the mixed mode), timed with HIP events on the
 solver's stream over the timed steps; roofline_xderiv2 does the same for the
-x-direction first+second derivative pass of T, u, v, w over a few further
-steps.  step_roofline puts
+x-direction first+second derivative pass of T, u, v, w (with d/dx of rho in
+the same launch) over a few further steps.  step_roofline puts
 the whole step against the same peak at its algorithmic bytes per cell-step
 (139 working-precision array passes, the FP64 state read and written, the RHS
 history: 1448 B in FP64, 780 B mixed).  cpu_baseline times
@@ -468,16 +468,20 @@ def measured_traffic(precision, kernel):
 
 
 # algorithmic bytes per cell of one launch of each timed kernel family
-#  xderiv2: 4 fields, 1 read + 2 writes each, working precision
+#  xderiv2: 4 fields, 1 read + 2 writes each, and rho, 1 read + 1 write
+#           (its first derivative only), working precision: 14 arrays
+#  yderiv2: the same along y with three single-output jobs (rho, d/dx u,
+#           d/dx v): 18 arrays
 #  hydro:   hydro_col with the fused AB3 update (interior and wall-row launches):
 #           reads the state (5), B and J^2 (4), the 26 x/y derivative arrays and
 #           history levels 1, 2 of 5 fields; writes 5 RHS and the 5 new state
 #           fields.  FP64: 55 x 8 = 440 B.  Mixed: FP32 state shadows, B, J^2,
 #           derivatives and history, plus the FP64 state read and written and
 #           the new FP32 shadows: 4 (5 + 4 + 26 + 10 + 5 + 5) + 8 (5 + 5) = 300 B.
-KERNEL_BYTES_PER_CELL = {"xderiv2": {8: 96, 4: 48}, "yderiv2": {8: 96, 4: 48}, "hydro": {8: 440, 4: 300}}
-KERNEL_DESC = {"xderiv2": "SpecK<T, plan, 0, 2, 1, 0> (xderiv2: d/dx and d2/dx2 of T, u, v, w)",
-               "yderiv2": "SpecK<T, plan, 1, B, 1, 0> (yderiv2: d/dy and d2/dy2 of T, u, v, w)",
+KERNEL_BYTES_PER_CELL = {"xderiv2": {8: 112, 4: 56}, "yderiv2": {8: 144, 4: 72}, "hydro": {8: 440, 4: 300}}
+KERNEL_DESC = {"xderiv2": "SpecK<T, plan, 0, 2, 1, 0> (xderiv2: d/dx and d2/dx2 of T, u, v, w; d/dx of rho)",
+               "yderiv2": "SpecK<T, plan, 1, B, 1, 0> (yderiv2: d/dy and d2/dy2 of T, u, v, w; d/dy of rho, "
+                          "d/dx u, d/dx v)",
                "hydro": "HydroColKernel<T, mixed, false> + <T, mixed, true> (hydro_col, interior and wall "
                         "rows): hydro_deriv's right-hand side after the x/y passes with the fused AB3 "
                         "update of u, v, w, rho, T"}

@@ -472,8 +472,10 @@ int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
         j.mean[0] = m0; j.mean[1] = m1; j.mean[2] = m2;
     };
     // x pass: d/dx, d2/dx2 of t, u, v, w (u's second output also carries the
-    // scko3 d/dx d/dx u of the composite flux, lines 181, 258); d/dx of r and
-    // of the Maxwell-stress products F by bx, F bz bx, F/2 (bx^2-by^2-bz^2)
+    // scko3 d/dx d/dx u of the composite flux, lines 181, 258) and d/dx of r,
+    // a single-output job of the same launch (one launch instead of two: at
+    // 256^2 x 160 FP64 242 against 210 + 47 us); then d/dx of the
+    // Maxwell-stress products F by bx, F bz bx, F/2 (bx^2-by^2-bz^2)
     {
         SpecLaunch L = spec_base(c, 0, SM_DERIV2);
         L.job[0] = sjob(T_, sf, W[W_TX], W[W_TXX], MUL_D1, MUL_D2, wf);
@@ -481,24 +483,26 @@ int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
         L.job[1].mc[0] = P.sck; L.job[1].mc[1] = P.scko3;
         L.job[2] = sjob(V_, sf, W[W_VX], W[W_VXX], MUL_D1, MUL_D2, wf);
         L.job[3] = sjob(W_, sf, W[W_WX], W[W_WXX], MUL_D1, MUL_D2, wf);
-        L.njobs = 4;
+        L.job[4] = sjob(R_, sf, W[W_RX], nullptr, MUL_D1, 0, wf);
+        L.njobs = 5;
         TRY(spec(c, L, MHD_K_XDERIV2));
         SpecLaunch M = spec_base(c, 0, SM_DERIV1);
-        M.job[0] = sjob(R_, sf, W[W_RX], nullptr, MUL_D1, 0, wf);
-        M.job[1] = sjob(c->by, wf, W[W_XBB], nullptr, MUL_D1, 0, wf);
+        M.job[0] = sjob(c->by, wf, W[W_XBB], nullptr, MUL_D1, 0, wf);
+        M.job[0].load_op = LD_PROD; M.job[0].in[1] = c->bx; M.job[0].scale = f;
+        means(M.job[0], mby, mbx, nullptr);
+        M.job[1] = sjob(c->bz, wf, W[W_XZX], nullptr, MUL_D1, 0, wf);
         M.job[1].load_op = LD_PROD; M.job[1].in[1] = c->bx; M.job[1].scale = f;
-        means(M.job[1], mby, mbx, nullptr);
-        M.job[2] = sjob(c->bz, wf, W[W_XZX], nullptr, MUL_D1, 0, wf);
-        M.job[2].load_op = LD_PROD; M.job[2].in[1] = c->bx; M.job[2].scale = f;
-        means(M.job[2], mbz, mbx, nullptr);
-        M.job[3] = sjob(c->bx, wf, W[W_DXM], nullptr, MUL_D1, 0, wf);
-        M.job[3].load_op = LD_MAXW; M.job[3].in[1] = c->by; M.job[3].in[2] = c->bz; M.job[3].scale = P.fo2;
-        means(M.job[3], mbx, mby, mbz);
-        M.njobs = 4;
+        means(M.job[1], mbz, mbx, nullptr);
+        M.job[2] = sjob(c->bx, wf, W[W_DXM], nullptr, MUL_D1, 0, wf);
+        M.job[2].load_op = LD_MAXW; M.job[2].in[1] = c->by; M.job[2].in[2] = c->bz; M.job[2].scale = P.fo2;
+        means(M.job[2], mbx, mby, mbz);
+        M.njobs = 3;
         TRY(spec(c, M));
     }
     // y pass: the same along y (v's second output carries scko3 d/dy d/dy v),
-    // plus the mixed derivatives d/dy(ux), d/dy(vx) of the composite fluxes
+    // with the single-output jobs d/dy of r and the mixed derivatives d/dy(ux),
+    // d/dy(vx) of the composite fluxes (FP64 318 against 219 + 120 us as two
+    // launches); then the Maxwell-stress products
     {
         SpecLaunch L = spec_base(c, 1, SM_DERIV2);
         L.job[0] = sjob(T_, sf, W[W_TY], W[W_TYY], MUL_D1, MUL_D2, wf);
@@ -506,22 +510,22 @@ int run_hydro_deriv(mhd_ctx* c, const double* cf = nullptr)
         L.job[2] = sjob(V_, sf, W[W_VY], W[W_VYY], MUL_D1, MUL_D2C, wf);
         L.job[2].mc[0] = P.sck; L.job[2].mc[1] = P.scko3;
         L.job[3] = sjob(W_, sf, W[W_WY], W[W_WYY], MUL_D1, MUL_D2, wf);
-        L.njobs = 4;
+        L.job[4] = sjob(R_, sf, W[W_RY], nullptr, MUL_D1, 0, wf);
+        L.job[5] = sjob(W[W_UX], wf, W[W_UXY], nullptr, MUL_D1, 0, wf);
+        L.job[6] = sjob(W[W_VX], wf, W[W_VXY], nullptr, MUL_D1, 0, wf);
+        L.njobs = 7;
         TRY(spec(c, L, MHD_K_YDERIV2));
         SpecLaunch M = spec_base(c, 1, SM_DERIV1);
-        M.job[0] = sjob(R_, sf, W[W_RY], nullptr, MUL_D1, 0, wf);
-        M.job[1] = sjob(W[W_UX], wf, W[W_UXY], nullptr, MUL_D1, 0, wf);
-        M.job[2] = sjob(W[W_VX], wf, W[W_VXY], nullptr, MUL_D1, 0, wf);
-        M.job[3] = sjob(c->by, wf, W[W_YBB], nullptr, MUL_D1, 0, wf);
-        M.job[3].load_op = LD_PROD; M.job[3].in[1] = c->bx; M.job[3].scale = f;
-        means(M.job[3], mby, mbx, nullptr);
-        M.job[4] = sjob(c->bz, wf, W[W_YZY], nullptr, MUL_D1, 0, wf);
-        M.job[4].load_op = LD_PROD; M.job[4].in[1] = c->by; M.job[4].scale = f;
-        means(M.job[4], mbz, mby, nullptr);
-        M.job[5] = sjob(c->by, wf, W[W_DYM], nullptr, MUL_D1, 0, wf);
-        M.job[5].load_op = LD_MAXW; M.job[5].in[1] = c->bx; M.job[5].in[2] = c->bz; M.job[5].scale = P.fo2;
-        means(M.job[5], mby, mbx, mbz);
-        M.njobs = 6;
+        M.job[0] = sjob(c->by, wf, W[W_YBB], nullptr, MUL_D1, 0, wf);
+        M.job[0].load_op = LD_PROD; M.job[0].in[1] = c->bx; M.job[0].scale = f;
+        means(M.job[0], mby, mbx, nullptr);
+        M.job[1] = sjob(c->bz, wf, W[W_YZY], nullptr, MUL_D1, 0, wf);
+        M.job[1].load_op = LD_PROD; M.job[1].in[1] = c->by; M.job[1].scale = f;
+        means(M.job[1], mbz, mby, nullptr);
+        M.job[2] = sjob(c->by, wf, W[W_DYM], nullptr, MUL_D1, 0, wf);
+        M.job[2].load_op = LD_MAXW; M.job[2].in[1] = c->bx; M.job[2].in[2] = c->bz; M.job[2].scale = P.fo2;
+        means(M.job[2], mby, mbx, mbz);
+        M.njobs = 3;
         TRY(spec(c, M));
     }
     HydroColArgs a = colargs(c);
@@ -1849,7 +1853,8 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
     double best_ms[2] = {0, 0};
     // transforms, per direction: the persistent-grid multiple on the hydro
     // d/dx, d2/dx2 set with the default plans, then per transform kind the
-    // radix plan on a 4-field launch of that kind (outputs into work arrays)
+    // radix plan on a launch of that kind shaped like the step's (outputs into
+    // work arrays)
     const int o1[2][4] = {{W_TX, W_UX, W_VX, W_WX}, {W_TY, W_UY, W_VY, W_WY}};
     const int o2[2][4] = {{W_TXX, W_UXX, W_VXX, W_WXX}, {W_TYY, W_UYY, W_VYY, W_WYY}};
     auto kind_launch = [&](int dir, int kind) {
@@ -1860,14 +1865,19 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
             else if (kind == SM_DERIV1) L.job[j] = sjob(in[j], sf, a, nullptr, MUL_D1, 0, wf);
             else L.job[j] = sjob(in[j], sf, a, nullptr, 0, 0, wf);
         }
-        if (kind == SM_DERIV1) {
-            // as in the step: a plain first derivative beside the three of one
-            // direction's Maxwell-stress loads (products of three fields)
-            L.job[1].in[0] = in[2]; L.job[1].load_op = LD_PROD; L.job[1].in[1] = in[1];
-            L.job[2].in[0] = in[3]; L.job[2].load_op = LD_PROD; L.job[2].in[1] = in[1];
-            L.job[3].in[0] = in[1]; L.job[3].load_op = LD_MAXW; L.job[3].in[1] = in[2]; L.job[3].in[2] = in[3];
-        }
         L.njobs = 4;
+        if (kind == SM_DERIV2) {
+            // as in the step: a single-output (first-derivative) job beside the four
+            L.job[4] = sjob(in[0], sf, W[dir == 0 ? W_RX : W_RY], nullptr, MUL_D1, 0, wf);
+            L.njobs = 5;
+        } else if (kind == SM_DERIV1) {
+            // as in the step: the first derivatives of one direction's three
+            // Maxwell-stress loads (products of three fields)
+            L.job[0].in[0] = in[2]; L.job[0].load_op = LD_PROD; L.job[0].in[1] = in[1];
+            L.job[1].in[0] = in[3]; L.job[1].load_op = LD_PROD; L.job[1].in[1] = in[1];
+            L.job[2].in[0] = in[1]; L.job[2].load_op = LD_MAXW; L.job[2].in[1] = in[2]; L.job[2].in[2] = in[3];
+            L.njobs = 3;
+        }
         return L;
     };
     for (int dir = 0; dir < 2; ++dir) {

@@ -55,7 +55,7 @@ struct SpecJob {
     const double* jcoef;
     double mc[2];     // MUL_D2C coefficients
     const double* bg; // optional per-plane value subtracted on load (nz entries)
-    void* out[2];
+    void* out[2];     // SM_DERIV2: out[1] may be null (the job's first derivative only)
     int out_f32;
     int mul[2];       // MulKind for out[0], out[1] (SM_DERIV*)
     const double* addk; // optional per-plane value added on store (nz entries)

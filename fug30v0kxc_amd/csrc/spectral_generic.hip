@@ -230,6 +230,7 @@ __device__ __forceinline__ void GenK<TC, DIR, MODE, LOP>::body(const SpecLaunch&
             } else {
                 constexpr int NOUT = (MODE == SM_DERIV2) ? 2 : 1;
                 for (int o = 0; o < NOUT; ++o) {
+                    if (o == 1 && out1 == nullptr) break;   // a first-derivative job (uniform)
                     // multiplied spectrum into `other`, inverse through buf2
                     for (int q = threadIdx.x; q < P; q += GNT) {
                         const int b = q / N, kk = q - b * N;

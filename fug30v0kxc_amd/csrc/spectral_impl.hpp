@@ -678,6 +678,7 @@ __device__ __forceinline__ void SpecK<TC, PL, DIR, B, MODE, LOP>::body_jobs(cons
                 }
 #pragma unroll
                 for (int o = 0; o < NOUT; ++o) {
+                    if (o == 1 && out1 == nullptr) break;   // a first-derivative job (uniform)
                     if (j < NBL) {
 #pragma unroll
                         for (int r = 0; r < RL; ++r)
