@@ -394,7 +394,7 @@ MagArgs margs(mhd_ctx* c)
     void** W = c->wk;
     // the RHS-assembly planes are free while the magnetic routines run
     a.exh = W[W_RX]; a.eyh = W[W_TX]; a.ezh = W[W_TXX];
-    a.d = W[W_UX]; a.cc = W[W_UXX]; a.dz = W[W_VX];
+    a.d = W[W_UX]; a.cc = W[W_UXX];
     a.polz = W[W_VXX];
     a.bxh = W[W_WX]; a.byh = W[W_WXX]; a.bzh = W[W_XBB]; a.jzh = W[W_XZX]; a.dybzh = W[W_RY]; a.dxbzh = W[W_TY];
     a.bx = c->bx; a.by = c->by; a.jsq = c->jsq;
@@ -628,12 +628,7 @@ int run_mag_deriv(mhd_ctx* c, const double* cf = nullptr)
     }
     TRY(kerr(launch_mean_emf(m, c->s)));
     TRY(kerr(launch_mag_spec1(m, c->s)));
-    {
-        ZLaunch Z = zbase(c);
-        Z.job[0] = zjob(m.d, wf, m.dz, wf, ZK_D1, 5);  // 419
-        Z.njobs = 1;
-        TRY(zrun(c, Z));
-    }
+    // D_z of d (line 419) is taken inside mag_rhs from a window of d
     if (cf) {
         m.ab3 = 1;
         m.c0 = cf[0]; m.c1 = cf[1]; m.c2 = cf[2];

@@ -287,7 +287,9 @@ __device__ __forceinline__ void MeanEmfKernel<TW>::body(const MagArgs& a, uint3 
 // poldot, tordot, src/magnetic.f90:429-479.  Column walk: a thread owns one
 // (i,j) column and a.zc planes; pol and tor stay in 5-plane register windows
 // so each plane is read once; the two planes next to each wall use the
-// one-sided d2bydz rows (flag 3) with direct loads.
+// one-sided d2bydz rows (flag 3) with direct loads.  D_z of d (line 419,
+// d1bydz flag 5) comes from a third window, in the working precision (as a
+// separate z pass would compute and store it), instead of a stored array.
 template <typename TW>
 struct MagRhsKernel {
     using Args = MagArgs;
@@ -320,40 +322,49 @@ __device__ __forceinline__ void MagRhsKernel<TW>::body(const MagArgs& a, uint3 b
     }
     const double* pol = a.pol;
     const double* tor = a.tor;
+    const TW* __restrict__ dd = (const TW*)a.d;
     const double zck = a.p.zck, s = a.p.sc1;
+    const TW hz = (TW)a.p.ho12;
     auto ld2 = [](const void* p, long q) { return *(const TW2*)((const TW*)p + q); };
     auto ldd = [](const double* p, long q) { return *(const D2*)(p + q); };
-    // interior planes [2, nz-2) use the centred stencil from the windows
+    // interior planes [2, nz-2) use the centred stencils from the windows
     const int ki0 = k0 > 2 ? k0 : 2, ki1 = k1 < nz - 2 ? k1 : nz - 2;
     double pw[2][5], tw[2][5];
+    TW dw[2][5];
     if (ki0 < ki1) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const long q = col + (long)(ki0 - 2 + j) * nxy;
             const D2 p2 = ldd(pol, q), t2 = ldd(tor, q);
+            const TW2 d2 = ld2(dd, q);
             pw[0][j + 1] = p2.x; pw[1][j + 1] = p2.y;
             tw[0][j + 1] = t2.x; tw[1][j + 1] = t2.y;
+            dw[0][j + 1] = d2.x; dw[1][j + 1] = d2.y;
         }
     }
     for (int k = k0; k < k1; ++k) {
         const long q = col + (long)k * nxy;
-        double pzz[2], tzz[2], p0[2], t0[2];
+        double pzz[2], tzz[2], p0[2], t0[2], dzv[2];
         if (k >= ki0 && k < ki1) {
             const D2 p2 = ldd(pol, q + 2L * nxy), t2 = ldd(tor, q + 2L * nxy);
+            const TW2 d2 = ld2(dd, q + 2L * nxy);
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     pw[c][j] = pw[c][j + 1];
                     tw[c][j] = tw[c][j + 1];
+                    dw[c][j] = dw[c][j + 1];
                 }
             }
             pw[0][4] = p2.x; pw[1][4] = p2.y;
             tw[0][4] = t2.x; tw[1][4] = t2.y;
+            dw[0][4] = d2.x; dw[1][4] = d2.y;
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 pzz[c] = s * (-pw[c][0] + 16.0 * pw[c][1] - 30.0 * pw[c][2] + 16.0 * pw[c][3] - pw[c][4]);
                 tzz[c] = s * (-tw[c][0] + 16.0 * tw[c][1] - 30.0 * tw[c][2] + 16.0 * tw[c][3] - tw[c][4]);
+                dzv[c] = (double)(hz * (dw[c][0] - (TW)8 * dw[c][1] + (TW)8 * dw[c][3] - dw[c][4]));
                 p0[c] = pw[c][2];
                 t0[c] = tw[c][2];
             }
@@ -362,13 +373,13 @@ __device__ __forceinline__ void MagRhsKernel<TW>::body(const MagArgs& a, uint3 b
             for (int c = 0; c < 2; ++c) {
                 pzz[c] = dz2<double>(k, nz, 3, s, [&](int kk) { return pol[col + c + (long)kk * nxy]; });
                 tzz[c] = dz2<double>(k, nz, 3, s, [&](int kk) { return tor[col + c + (long)kk * nxy]; });
+                dzv[c] = (double)dz1<TW>(k, nz, 5, hz, [&](int kk) { return dd[col + c + (long)kk * nxy]; });
                 p0[c] = pol[q + c];
                 t0[c] = tor[q + c];
             }
         }
-        const TW2 cc2 = ld2(a.cc, q), dz2v = ld2(a.dz, q), ez2 = ld2(a.ezh, q);
-        const double ccv[2] = {(double)cc2.x, (double)cc2.y}, dzv[2] = {(double)dz2v.x, (double)dz2v.y},
-                     ezv[2] = {(double)ez2.x, (double)ez2.y};
+        const TW2 cc2 = ld2(a.cc, q), ez2 = ld2(a.ezh, q);
+        const double ccv[2] = {(double)cc2.x, (double)cc2.y}, ezv[2] = {(double)ez2.x, (double)ez2.y};
         TW pd[2], td[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {

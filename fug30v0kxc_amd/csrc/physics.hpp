@@ -91,7 +91,7 @@ struct MagArgs {
     void *poldot, *tordot;        // newest history level (working precision)
     double *bxbardot, *bybardot;  // newest history level of the means (FP64)
     void *exh, *eyh, *ezh;        // spectral EMF / B (double-halfcomplex)
-    void *d, *cc, *dz;            // mag_deriv work
+    void *d, *cc;                 // mag_deriv work (mag_rhs takes D_z of d itself)
     void *polz;                   // bandj: d pol / dz
     void *bxh, *byh, *bzh, *jzh, *dybzh, *dxbzh;  // bandj spectral outputs
     const void *bx, *by;          // physical B for J^2
