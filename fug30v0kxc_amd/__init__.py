@@ -330,27 +330,28 @@ class Solver:
         return ms.value, n.value
 
     def autotune(self):
-        """Time this box's launch shapes (per transform direction the
-        persistent-grid multiple, and the radix plan of each transform kind;
-        the planes per thread of the hydro column kernel) and keep the
-        fastest; returns the choices and their times (variant_x, variant_y:
-        one plan per kind, d1, d1+d2, r2hc, hc2r)."""
+        """Time this box's launch shapes (per transform direction and kind the
+        radix plan and the persistent-grid multiple; the planes per thread of
+        the hydro column kernel) and keep the fastest; returns the choices and
+        their times (variant_x, variant_y, mult_x, mult_y: one per kind, d1,
+        d1+d2, r2hc, hc2r)."""
         t = _abi.MhdTune()
         self._chk(self._lib.mhd_autotune(self._ctx, ctypes.byref(t)), "autotune")
         out = {n: getattr(t, n) for n, _ in _abi.MhdTune._fields_}
-        out["variant_x"], out["variant_y"] = list(t.variant_x), list(t.variant_y)
+        for n in ("variant_x", "variant_y", "mult_x", "mult_y"):
+            out[n] = list(getattr(t, n))
         return out
 
     def set_tune(self, variant_x=0, variant_y=0, mult_x=1, mult_y=1, zc=8):
         """Apply launch-shape choices (as returned by autotune); a single
-        variant applies to every transform kind of its direction."""
+        variant or multiple applies to every transform kind of its direction."""
         def kinds(v):
             v = [v] * _abi.TUNE_KINDS if isinstance(v, int) else list(v)
             if len(v) != _abi.TUNE_KINDS:
-                raise ValueError(f"a variant per transform kind ({_abi.TUNE_KINDS}) or one for all")
+                raise ValueError(f"a value per transform kind ({_abi.TUNE_KINDS}) or one for all")
             return (ctypes.c_int * _abi.TUNE_KINDS)(*v)
-        t = _abi.MhdTune(variant_x=kinds(variant_x), variant_y=kinds(variant_y), mult_x=mult_x,
-                         mult_y=mult_y, zc=zc)
+        t = _abi.MhdTune(variant_x=kinds(variant_x), variant_y=kinds(variant_y), mult_x=kinds(mult_x),
+                         mult_y=kinds(mult_y), zc=zc)
         self._chk(self._lib.mhd_set_tune(self._ctx, ctypes.byref(t)), "set_tune")
 
     def sync(self):

@@ -79,8 +79,8 @@ TUNE_KINDS = 4   # radix-plan choices per direction: d1, d1+d2, r2hc, hc2r (MHD_
 
 
 class MhdTune(ctypes.Structure):
-    _fields_ = ([("variant_x", ctypes.c_int * TUNE_KINDS), ("variant_y", ctypes.c_int * TUNE_KINDS)]
-                + [(n, ctypes.c_int) for n in ("mult_x", "mult_y", "zc")]
+    _fields_ = ([(n, ctypes.c_int * TUNE_KINDS) for n in ("variant_x", "variant_y", "mult_x", "mult_y")]
+                + [("zc", ctypes.c_int)]
                 + [(n, ctypes.c_double) for n in ("ms_x", "ms_y", "ms_col")])
 
 

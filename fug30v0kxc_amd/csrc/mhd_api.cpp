@@ -101,7 +101,7 @@ struct mhd_ctx {
     double* stage;     // device FP64 staging (one field)
     int ncu;           // compute units (persistent transform grids)
     int spec_var[2][4]; // radix plan per direction and transform kind (SpecMode), tuned
-    int spec_mult[2];  // persistent-grid multiple per direction, tuned
+    int spec_mult[2][4];  // persistent-grid multiple per direction and SpecMode, tuned
     int hzc;           // planes per thread of the column-walking hydro kernel
     int mzc;           // ... of the magnetic column kernels (mag_rhs, jsq)
     double* hbuf;      // pinned host staging (one field)
@@ -252,7 +252,7 @@ SpecLaunch spec_base(mhd_ctx* c, int dir, int mode, bool f64 = false)
     L.icut = dir == 0 ? c->icutx : c->icuty;
     L.ncu = c->ncu;
     L.variant = c->spec_var[dir][mode];
-    L.grid_mult = c->spec_mult[dir];
+    L.grid_mult = c->spec_mult[dir][mode];
     return L;
 }
 
@@ -1023,8 +1023,10 @@ int mhd_create(const mhd_params_t* p, int nx, int ny, int nz, int precision, mhd
     // 256x256x160 on MI355X (y transforms with two resident grids in FP64; 8
     // planes per column-walk thread)
     for (int k = 0; k < 4; ++k) c->spec_var[0][k] = c->spec_var[1][k] = 0;
-    c->spec_mult[0] = 1;
-    c->spec_mult[1] = c->wf32 ? 1 : 2;
+    for (int k = 0; k < 4; ++k) {
+        c->spec_mult[0][k] = 1;
+        c->spec_mult[1][k] = c->wf32 ? 1 : 2;
+    }
     {
         const int nin = c->nz > 8 ? c->nz - 8 : 1;   // interior planes
         c->hzc = 8 < nin ? 8 : nin;
@@ -1823,13 +1825,15 @@ int mhd_set_tune(mhd_ctx_t* c, const mhd_tune_t* t)
         if (t->variant_x[k] < 0 || t->variant_y[k] < 0 || t->variant_x[k] >= spectral_variants(c->nx) ||
             t->variant_y[k] >= spectral_variants(c->ny))
             return MHD_E_ARG;
-    if (t->mult_x < 1 || t->mult_x > 8 || t->mult_y < 1 || t->mult_y > 8 || t->zc < 1) return MHD_E_ARG;
+    for (int k = 0; k < MHD_TK_N; ++k)
+        if (t->mult_x[k] < 1 || t->mult_x[k] > 8 || t->mult_y[k] < 1 || t->mult_y[k] > 8) return MHD_E_ARG;
+    if (t->zc < 1) return MHD_E_ARG;
     for (int k = 0; k < MHD_TK_N; ++k) {
         c->spec_var[0][k] = t->variant_x[k];
         c->spec_var[1][k] = t->variant_y[k];
+        c->spec_mult[0][k] = t->mult_x[k];
+        c->spec_mult[1][k] = t->mult_y[k];
     }
-    c->spec_mult[0] = t->mult_x;
-    c->spec_mult[1] = t->mult_y;
     c->hzc = t->zc;
     return 0;
 }
@@ -1846,10 +1850,10 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
                          c->su ? (const void*)c->sv : c->v, c->su ? (const void*)c->sw : c->w};
     const int sf = c->su ? 1 : 0;
     double best_ms[2] = {0, 0};
-    // transforms, per direction: the persistent-grid multiple on the hydro
-    // d/dx, d2/dx2 set with the default plans, then per transform kind the
-    // radix plan on a launch of that kind shaped like the step's (outputs into
-    // work arrays)
+    // transforms, per direction and kind: the radix plan and the
+    // persistent-grid multiple, on a launch of that kind shaped like the
+    // step's (outputs into work arrays); the current choice (the default at
+    // first) is timed first
     const int o1[2][4] = {{W_TX, W_UX, W_VX, W_WX}, {W_TY, W_UY, W_VY, W_WY}};
     const int o2[2][4] = {{W_TXX, W_UXX, W_VXX, W_WXX}, {W_TYY, W_UYY, W_VYY, W_WYY}};
     auto kind_launch = [&](int dir, int kind) {
@@ -1877,28 +1881,23 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
     };
     for (int dir = 0; dir < 2; ++dir) {
         const int n = dir == 0 ? c->nx : c->ny;
-        for (int k = 0; k < MHD_TK_N; ++k) c->spec_var[dir][k] = 0;
-        double best = 1e30;
-        int bm = 1;
-        for (int mult = 1; mult <= 2; ++mult) {
-            c->spec_mult[dir] = mult;
-            const SpecLaunch L = kind_launch(dir, SM_DERIV2);
-            double ms = 0.0;
-            TRY(time_ms(c, TUNE_REPS, [&] { return spec(c, L); }, &ms));
-            if (ms < TUNE_GAIN * best) { best = ms; bm = mult; }
-        }
-        c->spec_mult[dir] = bm;
+        const int mdef = (dir == 1 && !c->wf32) ? 2 : 1;
         for (int kind = 0; kind < MHD_TK_N; ++kind) {
             double bk = 1e30;
-            int bv = 0;
+            int bv = 0, bm = mdef;
             for (int var = 0; var < spectral_variants(n); ++var) {
-                c->spec_var[dir][kind] = var;
-                const SpecLaunch L = kind_launch(dir, kind);
-                double ms = 0.0;
-                TRY(time_ms(c, TUNE_REPS, [&] { return spec(c, L); }, &ms));
-                if (ms < TUNE_GAIN * bk) { bk = ms; bv = var; }
+                for (int mi = 0; mi < 2; ++mi) {
+                    const int mult = mi == 0 ? mdef : 3 - mdef;
+                    c->spec_var[dir][kind] = var;
+                    c->spec_mult[dir][kind] = mult;
+                    const SpecLaunch L = kind_launch(dir, kind);
+                    double ms = 0.0;
+                    TRY(time_ms(c, TUNE_REPS, [&] { return spec(c, L); }, &ms));
+                    if (ms < TUNE_GAIN * bk) { bk = ms; bv = var; bm = mult; }
+                }
             }
             c->spec_var[dir][kind] = bv;
+            c->spec_mult[dir][kind] = bm;
             if (kind == SM_DERIV2) best_ms[dir] = bk;
         }
     }
@@ -1929,8 +1928,10 @@ int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out)
             out->variant_x[k] = c->spec_var[0][k];
             out->variant_y[k] = c->spec_var[1][k];
         }
-        out->mult_x = c->spec_mult[0];
-        out->mult_y = c->spec_mult[1];
+        for (int k = 0; k < MHD_TK_N; ++k) {
+            out->mult_x[k] = c->spec_mult[0][k];
+            out->mult_y[k] = c->spec_mult[1][k];
+        }
         out->zc = c->hzc;
         out->ms_x = best_ms[0];
         out->ms_y = best_ms[1];

@@ -199,11 +199,11 @@ long mhd_dump_header(const mhd_dump_t* meta, int iprec, unsigned char* buf, long
 
 /* ---- autotuner (no counterpart in the reference) ---------------------------
  * Times this box's launch shapes on this GPU and keeps the fastest in the
- * context: per transform direction the persistent-grid multiple (1 or 2 x
- * resident, on the hydro d/dx, d2/dx2 launch) and, per transform kind (first
- * derivatives, first and second derivatives, real to halfcomplex and back),
- * the radix plan (the default or one of the length's alternatives, on a
- * launch of that kind); the planes per thread of the column-walking
+ * context: per transform direction and kind (first derivatives, first and
+ * second derivatives, real to halfcomplex and back) the radix plan (the
+ * default or one of the length's alternatives) together with the
+ * persistent-grid multiple (1 or 2 x resident), on a launch of that kind
+ * shaped like the step's; the planes per thread of the column-walking
  * hydro kernel, in the variant the step runs (with the fused AB3 update
  * unless a closure is attached; the magnetic column kernels keep 8 planes).
  * Needs a state; overwrites only work arrays, the buffers the next step's
@@ -214,7 +214,9 @@ typedef struct mhd_tune {
     /* radix plan per transform kind (index MHD_TK_*): 0 the default, 1 and 2
      * the length's alternatives where it has them */
     int variant_x[MHD_TK_N], variant_y[MHD_TK_N];
-    int mult_x, mult_y, zc;
+    /* persistent-grid multiple per transform kind (1 .. 8) */
+    int mult_x[MHD_TK_N], mult_y[MHD_TK_N];
+    int zc;
     double ms_x, ms_y, ms_col;   /* the tuned d/dx+d2/dx2, d/dy+d2/dy2 and hydro column launches */
 } mhd_tune_t;
 int mhd_autotune(mhd_ctx_t* c, mhd_tune_t* out);

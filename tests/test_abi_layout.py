@@ -21,7 +21,7 @@ STRUCTS = {
     "mhd_scalars_t": (_abi.MhdScalars, ["dt", "bzbar", "nit", "second"]),
     "mhd_diag_t": (_abi.MhdDiag, ["ubar", "divbmin", "dt", "nit"]),
     "mhd_dump_t": (_abi.MhdDump, ["nx", "iteration", "time", "timestep", "gamma", "ymax", "history"]),
-    "mhd_tune_t": (_abi.MhdTune, ["variant_x", "zc", "ms_x", "ms_col"]),
+    "mhd_tune_t": (_abi.MhdTune, ["variant_x", "mult_x", "mult_y", "zc", "ms_x", "ms_col"]),
 }
 
 

@@ -45,7 +45,7 @@ def test_autotune_then_steps_match_oracle():
     s.static()
     t = s.autotune()
     assert all(v in (0, 1, 2) for v in t["variant_x"] + t["variant_y"])
-    assert t["mult_x"] in (1, 2) and t["zc"] >= 8
+    assert all(m in (1, 2) for m in t["mult_x"] + t["mult_y"]) and t["zc"] >= 8
     assert t["ms_x"] > 0 and t["ms_col"] > 0
     o = Oracle(64, 64, 48, p)
     o.setup_static()
@@ -89,7 +89,8 @@ def test_forced_alternative_steps_match_oracle():
     o.step(12, 1e-4, 0)
     for prec, tol in ((0, 1e-10), (MHD_PREC_MIXED, 1e-5)):
         s = Solver(64, 64, 48, p, precision=prec)
-        s.set_tune(variant_x=1, variant_y=1, mult_x=1, mult_y=2, zc=12)
+        # a grid multiple per transform kind, as the tuner may choose
+        s.set_tune(variant_x=1, variant_y=1, mult_x=[1, 2, 2, 1], mult_y=[2, 1, 1, 2], zc=12)
         s.static()
         s.step(12, 1e-4, 0)
         st = s.get_state()
@@ -148,6 +149,8 @@ def test_set_tune_rejects_invalid_choices():
         s.set_tune(variant_x=1)
     with pytest.raises(MhdError):
         s.set_tune(mult_x=0)
+    with pytest.raises(MhdError):
+        s.set_tune(mult_y=[1, 2, 9, 1])
     s.close()
     s = Solver(128, 256, 8, reference_params())       # 128: one alternative; 256: two
     s.set_tune(variant_x=1, variant_y=2)
